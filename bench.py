@@ -1,0 +1,176 @@
+"""Benchmark of the D3Q19 TRT step on one MI355X.
+
+Workload (one "step" = one lattice update of every fluid cell):
+  512^3 channel (walls at y = 0 and y = 511, periodic x and z) with 2000
+  seeded spheres (radius U[4,12]), aa-soa, omega 1.25 (tau 0.8),
+  Lambda 3/16, body force g = 1e-6, start f_i = w_i (1 + eps),
+  eps ~ U(-1e-3, 1e-3) from seed 0.  Inputs are built on the device; the
+  timed region holds only step launches (hipEvents on the library's stream).
+
+Printed: one JSON line.  value = FP64 MLUPS (fluid cells x steps / s);
+roofline = 304 B per fluid-cell update over the measured step time against
+8 TB/s; fp16 = the same workload with FP16 shifted-deviation storage
+(76 B per update) and its error against the FP64 run; cpu_baseline = the
+reference's own aa-soa (oracle/_ref/lbmbench-ref, serial as in the
+reference) on a 256^3 channel on this host.
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "lbm-bench_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "MLUPS + %HBM roofline, D3Q19 TRT 512\u00b3 FP64 vs FP16-store; adjoint MLUPS"
+BYTES_PER_LUP = {"native": 304, "fp32": 152, "fp16": 76, "bf16": 76}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(kernel_tag):
+    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/*.md)"""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    return data.get(kernel_tag)
+
+
+def cpu_baseline(seconds_hint):
+    """LBM-Bench's own aa-soa (serial in the reference) on a 256^3 channel"""
+    exe = os.path.join(ROOT, "oracle", "_ref", "lbmbench-ref")
+    if not os.path.exists(exe):
+        return None
+    iters = 24
+    cmd = [exe, "-d", "256x256x256", "-g", "channel", "-z", "-k", "aa-soa", "-i", str(iters),
+           "-o", "1.25", "-f", "1e-6"]
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    log("cpu baseline: " + " ".join(cmd[1:]))
+    out = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600).stdout
+    m = re.search(r"^P:\s+([0-9.]+) MFLUP/s\s+d: ([0-9.]+) s", out, flags=re.M)
+    if not m:
+        return None
+    return {"value": float(m.group(1)), "unit": "MLUPS", "cores": 1, "kind": "reference",
+            "sample": f"LBM-Bench aa-soa (serial in the reference), 256^3 channel -z, {iters} "
+                      f"steps, {float(m.group(2)):.1f} s; the reference cannot index 512^3"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--spheres", type=int, default=2000)
+    ap.add_argument("--kernel", default="aa-soa")
+    ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the halo-exchange path across GPUs is not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import lbmhip
+    lib = lbmhip.Library("dp")
+    dev, cus, mem = lib.device_info()
+    log(f"device {dev}, {cus} CUs, {mem / 2**30:.0f} GiB")
+
+    n = args.size
+    dims = (n, n, n)
+    geo = f"spheres-{args.spheres}"
+    kw = dict(omega=1.25, xforce=1e-6, body_force=True, magic=3.0 / 16.0, seed=0)
+
+    copy_gbs = lib.copy_bandwidth(4 << 30, 10)
+    log(f"device copy bandwidth {copy_gbs:.0f} GB/s")
+
+    result = {}
+    L64 = lbmhip.Lattice(lib, args.kernel, geo, dims, (0, 0, 1), **kw)
+    nfluid = L64.n_fluid
+    solid_fraction = 1.0 - nfluid / float(n ** 3)
+    log(f"lattice {dims}, {nfluid} fluid cells, solid fraction {solid_fraction:.4f}, "
+        f"{L64.device_bytes() / 2**30:.1f} GiB of PDFs")
+    L64.init("random", 0, 1.0, 1e-3)
+    m0 = L64.mass_momentum()
+    L64.run(args.warmup)
+    lib.synchronize()
+    t0 = time.perf_counter()
+    dur = L64.run(args.steps)
+    lib.synchronize()
+    wall = time.perf_counter() - t0
+    m1 = L64.mass_momentum()
+    mlups = nfluid * args.steps / dur / 1e6
+    step_s = dur / args.steps
+    achieved = BYTES_PER_LUP["native"] * nfluid / step_s / 1e9
+    log(f"fp64 {args.kernel}: {mlups:.0f} MLUPS, {step_s * 1e3:.3f} ms/step, "
+        f"{achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS * 100:.1f} % of 8 TB/s), wall {wall:.2f} s")
+
+    fp16 = None
+    if not args.no_fp16:
+        L16 = lbmhip.Lattice(lib, args.kernel, geo, dims, (0, 0, 1), storage="fp16", **kw)
+        L16.init("random", 0, 1.0, 1e-3)
+        L16.run(args.warmup)
+        lib.synchronize()
+        d16 = L16.run(args.steps)
+        m16 = L16.mass_momentum()
+        cmp = L16.compare_momentum(L64)
+        ml16 = nfluid * args.steps / d16 / 1e6
+        ach16 = BYTES_PER_LUP["fp16"] * nfluid / (d16 / args.steps) / 1e9
+        fp16 = {"mlups": round(ml16, 1), "speedup_vs_fp64": round(ml16 / mlups, 3),
+                "ms_per_step": round(d16 / args.steps * 1e3, 4),
+                "roofline": {"bound": "hbm", "achieved": round(ach16, 1), "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": round(ach16 / HBM_PEAK_GBS, 4),
+                             "bytes_per_lup": BYTES_PER_LUP["fp16"]},
+                "velocity_l2_rel_err_vs_fp64": (cmp[0] / cmp[1]) ** 0.5 if cmp[1] > 0 else None,
+                "velocity_max_abs_err_vs_fp64": cmp[2],
+                "mass_drift_rel": (m16[0] - m0[0]) / m0[0],
+                "steps_compared": args.warmup + args.steps}
+        log(f"fp16 {args.kernel}: {ml16:.0f} MLUPS ({ml16 / mlups:.2f}x fp64), "
+            f"u L2 err {fp16['velocity_l2_rel_err_vs_fp64']:.3e}")
+        L16.close()
+    L64.close()
+
+    cpu = None if args.no_cpu else cpu_baseline(20)
+
+    out = {
+        "metric": METRIC,
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_s * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded sphere packing and seeded f_i = w_i(1+eps), built on the device",
+        "config": {"workload": f"D3Q19 TRT {args.kernel}, {n}^3 channel (walls at y) with "
+                               f"{args.spheres} seeded spheres r~U[4,12], omega 1.25, Lambda 3/16, "
+                               f"body force 1e-6", "kernel": args.kernel, "lattice": list(dims),
+                   "fluid_cells": nfluid, "solid_fraction": round(solid_fraction, 5),
+                   "storage": "fp64"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles(f"{args.kernel}-fp64-{n}")},
+        "cpu_baseline": cpu,
+        "fp16": fp16,
+        "adjoint_mlups": None,
+        "mass_drift_rel": (m1[0] - m0[0]) / m0[0],
+        "copy_bandwidth_gbs": round(copy_gbs, 1),
+        "device": dev,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,444 @@
+// lbm_aux.hip -- passes around the step: body force at the end of a run,
+// seeded initialisation, FP64 moment reductions, field comparison and a
+// plain copy kernel.  One thread per cell (cell = (x*nY + y)*nZ + z); they
+// run once per run or per check, not per step, so the logical-slot rule and
+// layout are runtime arguments here.
+#include "lbm_impl.h"
+
+namespace lbm {
+
+__device__ __forceinline__ double weightOf(int d)
+{
+  return d == C ? 1.0 / 3.0 : (wclass(d) == 1 ? 1.0 / 18.0 : 1.0 / 36.0);
+}
+
+// slot holding logical f_d of cell (x, y, z) under rule (src/lbm.c:193-294
+// restated for the masked, halo-free layout)
+__device__ __forceinline__ uint64_t logicalSlot(const Geom &g, int rule, int x, int y, int z,
+    uint64_t cell, int d)
+{
+  auto sl = [&](uint64_t c, int k) -> uint64_t {
+    return g.aos ? c * Q + (uint64_t)k : (uint64_t)k * g.plane + c;
+  };
+  if (rule == SR_LOCAL || d == C)
+    return sl(cell, d);
+  const int xn = wrapAdd(x, -cx(d), g.nx), yn = wrapAdd(y, -cy(d), g.ny),
+            zn = wrapAdd(z, -cz(d), g.nz);
+  const bool bb    = solidAt(g.mask, g.segs, g.ny, xn, yn, zn);
+  const uint64_t n = ((uint64_t)xn * g.ny + yn) * g.nz + zn;
+  if (rule == SR_PULL)
+    return bb ? sl(cell, inv(d)) : sl(n, d);
+  return bb ? sl(cell, d) : sl(n, inv(d));
+}
+
+struct CellPos {
+  int x, y, z;
+  uint64_t cell;
+  bool valid;
+};
+__device__ __forceinline__ CellPos cellPos(const Geom &g)
+{
+  CellPos c;
+  const uint64_t n = (uint64_t)g.nx * g.ny * g.nz;
+  c.cell           = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  c.valid          = c.cell < n;
+  if (!c.valid)
+    return c;
+  c.z              = (int)(c.cell % g.nz);
+  const uint64_t r = c.cell / g.nz;
+  c.y              = (int)(r % g.ny);
+  c.x              = (int)(r / g.ny);
+  c.valid          = !solidAt(g.mask, g.segs, g.ny, c.x, c.y, c.z);
+  return c;
+}
+
+// ---- body force pass -------------------------------------------------------
+template <typename S>
+__global__ __launch_bounds__(256) void kForce(S *a, Geom g, int rule, float scale, float inv,
+    double f1, double f2)
+{
+  using R  = typename Store<S>::R;
+  using St = Store<S>;
+  const CellPos c = cellPos(g);
+  if (!c.valid)
+    return;
+#pragma unroll
+  for (int d = 0; d < Q; ++d) {
+    if (cx(d) == 0)
+      continue;
+    const R fd       = (R)(wclass(d) == 1 ? f1 : f2);
+    const uint64_t i = logicalSlot(g, rule, c.x, c.y, c.z, c.cell, d);
+    R v              = St::load(a[i], (R)inv);
+    v += cx(d) > 0 ? fd : -fd;
+    a[i] = St::store(v, (R)scale);
+  }
+}
+
+void launchForce(const AuxLaunch &A, double f1, double f2, hipStream_t s)
+{
+  const uint64_t n      = (uint64_t)A.g.nx * A.g.ny * A.g.nz;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+#define LAUNCH(T)                                                                          \
+  hipLaunchKernelGGL(kForce<T>, dim3(blocks), dim3(256), 0, s, (T *)A.pdf, A.g, A.rule, A.scale, \
+      A.invScale, f1, f2)
+  switch (A.storage) {
+  case ST_F64: LAUNCH(double); break;
+  case ST_F32: LAUNCH(float); break;
+  case ST_F32DEV: LAUNCH(f32dev); break;
+  case ST_F16: LAUNCH(__half); break;
+  default: LAUNCH(__hip_bfloat16); break;
+  }
+#undef LAUNCH
+}
+
+// ---- seeded initialisation -------------------------------------------------
+// splitmix64 finalizer (Steele, Lea & Flood 2014) chained over (seed, a, b)
+__device__ __forceinline__ uint64_t mix64(uint64_t x)
+{
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+__device__ __forceinline__ uint64_t hash3(uint64_t seed, uint64_t a, uint64_t b)
+{
+  return mix64(mix64(mix64(seed) ^ a) ^ b);
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void kInit(S *a, Geom g, int rule, int mode, uint64_t seed,
+    double rho, double amp, double rhoRef, float scale)
+{
+#pragma clang fp contract(off)
+  using R  = typename Store<S>::R;
+  using St = Store<S>;
+  const CellPos c = cellPos(g);
+  if (!c.valid)
+    return;
+  const uint64_t lcell = c.cell; // logical cell index: same ordering
+  for (int d = 0; d < Q; ++d) {
+    double e = 0.0;
+    if (mode == 2) {
+      const double u = (double)(hash3(seed, lcell, (uint64_t)d) >> 11) * (1.0 / 9007199254740992.0);
+      e              = (2.0 * u - 1.0) * amp;
+    }
+    double v;
+    if (St::scaled)
+      v = mode == 0 ? -weightOf(d) * rhoRef : weightOf(d) * ((rho - rhoRef) + e);
+    else
+      v = mode == 0 ? 0.0 : weightOf(d) * (rho + e);
+    const uint64_t i = logicalSlot(g, rule, c.x, c.y, c.z, c.cell, d);
+    a[i]             = St::store((R)v, (R)scale);
+  }
+}
+
+void launchInit(const AuxLaunch &A, int mode, uint64_t seed, double rho, double amp,
+    hipStream_t s)
+{
+  const uint64_t n      = (uint64_t)A.g.nx * A.g.ny * A.g.nz;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+#define LAUNCH(T)                                                                          \
+  hipLaunchKernelGGL(kInit<T>, dim3(blocks), dim3(256), 0, s, (T *)A.pdf, A.g, A.rule, mode, seed, \
+      rho, amp, A.rhoRef, A.scale)
+  switch (A.storage) {
+  case ST_F64: LAUNCH(double); break;
+  case ST_F32: LAUNCH(float); break;
+  case ST_F32DEV: LAUNCH(f32dev); break;
+  case ST_F16: LAUNCH(__half); break;
+  default: LAUNCH(__hip_bfloat16); break;
+  }
+#undef LAUNCH
+}
+
+// ---- logical gather / scatter ------------------------------------------------
+template <typename S>
+__global__ __launch_bounds__(256) void kGather(const S *a, Geom g, int rule, double rhoRef,
+    float inv, double *out)
+{
+  using R  = typename Store<S>::R;
+  using St = Store<S>;
+  const CellPos c  = cellPos(g);
+  const uint64_t n = (uint64_t)g.nx * g.ny * g.nz;
+  if (c.cell >= n)
+    return;
+  for (int d = 0; d < Q; ++d) {
+    double v = 0.0;
+    if (c.valid) {
+      const uint64_t i = logicalSlot(g, rule, c.x, c.y, c.z, c.cell, d);
+      v = St::scaled ? weightOf(d) * rhoRef + (double)St::load(a[i], (R)inv)
+                     : (double)St::load(a[i], (R)inv);
+    }
+    out[c.cell * Q + d] = v;
+  }
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void kScatter(S *a, Geom g, int rule, double rhoRef,
+    float scale, const double *in)
+{
+#pragma clang fp contract(off)
+  using R  = typename Store<S>::R;
+  using St = Store<S>;
+  const CellPos c = cellPos(g);
+  if (!c.valid)
+    return;
+  for (int d = 0; d < Q; ++d) {
+    const double v   = in[c.cell * Q + d];
+    const uint64_t i = logicalSlot(g, rule, c.x, c.y, c.z, c.cell, d);
+    a[i] = St::scaled ? St::store((R)(v - weightOf(d) * rhoRef), (R)scale) : St::store((R)v, (R)scale);
+  }
+}
+
+void launchGather(const AuxLaunch &A, double *out, hipStream_t s)
+{
+  const uint64_t n      = (uint64_t)A.g.nx * A.g.ny * A.g.nz;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+#define LAUNCH(T)                                                                          \
+  hipLaunchKernelGGL(kGather<T>, dim3(blocks), dim3(256), 0, s, (const T *)A.pdf, A.g, A.rule,    \
+      A.rhoRef, A.invScale, out)
+  switch (A.storage) {
+  case ST_F64: LAUNCH(double); break;
+  case ST_F32: LAUNCH(float); break;
+  case ST_F32DEV: LAUNCH(f32dev); break;
+  case ST_F16: LAUNCH(__half); break;
+  default: LAUNCH(__hip_bfloat16); break;
+  }
+#undef LAUNCH
+}
+
+void launchScatter(const AuxLaunch &A, const double *in, hipStream_t s)
+{
+  const uint64_t n      = (uint64_t)A.g.nx * A.g.ny * A.g.nz;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+#define LAUNCH(T)                                                                          \
+  hipLaunchKernelGGL(kScatter<T>, dim3(blocks), dim3(256), 0, s, (T *)A.pdf, A.g, A.rule,         \
+      A.rhoRef, A.scale, in)
+  switch (A.storage) {
+  case ST_F64: LAUNCH(double); break;
+  case ST_F32: LAUNCH(float); break;
+  case ST_F32DEV: LAUNCH(f32dev); break;
+  case ST_F16: LAUNCH(__half); break;
+  default: LAUNCH(__hip_bfloat16); break;
+  }
+#undef LAUNCH
+}
+
+// ---- FP64 reductions -------------------------------------------------------
+// wavefront sum by xor-shuffles (DPP / ds_swizzle lane exchange), then
+// the four wavefronts of the workgroup through LDS, in a fixed order
+__device__ __forceinline__ double waveSum(double v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+    v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double waveMax(double v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+    v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// reduce k values per thread across a 256-lane workgroup; lane 0 of
+// wavefront 0 ends with the totals.  isMax selects max instead of sum
+// for value index kmax (-1: none).
+template <int K>
+__device__ __forceinline__ void blockReduce(double (&v)[K], int kmax)
+{
+  __shared__ double part[4][K];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    v[k] = k == kmax ? waveMax(v[k]) : waveSum(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      part[w][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double t = part[0][k];
+      for (int j = 1; j < 4; ++j)
+        t = k == kmax ? fmax(t, part[j][k]) : t + part[j][k];
+      v[k] = t;
+    }
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void kMoments(const S *a, Geom g, int rule, float inv,
+    double *partials, double *field)
+{
+  using R  = typename Store<S>::R;
+  using St = Store<S>;
+  const CellPos c = cellPos(g);
+  double m[4]     = { 0.0, 0.0, 0.0, 0.0 };
+  const uint64_t n = (uint64_t)g.nx * g.ny * g.nz;
+  if (c.valid) {
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+      const uint64_t i = logicalSlot(g, rule, c.x, c.y, c.z, c.cell, d);
+      const double v   = (double)St::load(a[i], (R)inv);
+      m[0] += v;
+      m[1] += cx(d) * v;
+      m[2] += cy(d) * v;
+      m[3] += cz(d) * v;
+    }
+  }
+  if (field != nullptr && c.cell < n) {
+    const double nan = __builtin_nan("");
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      field[k * n + c.cell] = c.valid ? m[k] : nan;
+  }
+  blockReduce<4>(m, -1);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      partials[4 * (uint64_t)blockIdx.x + k] = m[k];
+}
+
+int momentsBlocks(const Geom &g)
+{
+  const uint64_t n = (uint64_t)g.nx * g.ny * g.nz;
+  return (int)((n + 255) / 256);
+}
+
+int launchMoments(const AuxLaunch &A, double *partials, double *field, hipStream_t s)
+{
+  const int blocks = momentsBlocks(A.g);
+#define LAUNCH(T)                                                                          \
+  hipLaunchKernelGGL(kMoments<T>, dim3(blocks), dim3(256), 0, s, (const T *)A.pdf, A.g, A.rule,   \
+      A.invScale, partials, field)
+  switch (A.storage) {
+  case ST_F64: LAUNCH(double); break;
+  case ST_F32: LAUNCH(float); break;
+  case ST_F32DEV: LAUNCH(f32dev); break;
+  case ST_F16: LAUNCH(__half); break;
+  default: LAUNCH(__hip_bfloat16); break;
+  }
+#undef LAUNCH
+  return blocks;
+}
+
+// final pass: one workgroup sums the partials in a fixed order
+template <int K>
+__global__ __launch_bounds__(256) void kReduceK(const double *partials, int n, double *out,
+    int kmax)
+{
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    v[k] = k == kmax ? 0.0 : 0.0;
+  for (int i = threadIdx.x; i < n; i += 256)
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      v[k] = k == kmax ? fmax(v[k], partials[(uint64_t)K * i + k]) : v[k] + partials[(uint64_t)K * i + k];
+  blockReduce<K>(v, kmax);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      out[k] = v[k];
+}
+
+void launchReduce4(const double *partials, int n, double *out, hipStream_t s)
+{
+  hipLaunchKernelGGL(kReduceK<4>, dim3(1), dim3(256), 0, s, partials, n, out, -1);
+}
+
+// {sum |jA - jB|^2, sum |jB|^2, max |jA - jB|, sum (rhoA - rhoB)^2} over
+// cells where both fields hold numbers
+__global__ __launch_bounds__(256) void kCompare(const double *fa, const double *fb, uint64_t n,
+    double *partials)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v[4]      = { 0.0, 0.0, 0.0, 0.0 };
+  if (i < n && !__builtin_isnan(fa[i]) && !__builtin_isnan(fb[i])) {
+    const double dx = fa[n + i] - fb[n + i], dy = fa[2 * n + i] - fb[2 * n + i],
+                 dz = fa[3 * n + i] - fb[3 * n + i];
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    v[0]            = d2;
+    v[1] = fb[n + i] * fb[n + i] + fb[2 * n + i] * fb[2 * n + i] + fb[3 * n + i] * fb[3 * n + i];
+    v[2] = sqrt(d2);
+    v[3] = (fa[i] - fb[i]) * (fa[i] - fb[i]);
+  }
+  blockReduce<4>(v, 2);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      partials[4 * (uint64_t)blockIdx.x + k] = v[k];
+}
+
+int launchCompareFields(const double *fa, const double *fb, uint64_t n, double *partials,
+    hipStream_t s)
+{
+  const int blocks = (int)((n + 255) / 256);
+  hipLaunchKernelGGL(kCompare, dim3(blocks), dim3(256), 0, s, fa, fb, n, partials);
+  return blocks;
+}
+
+void launchReduceCompare(const double *partials, int n, double *out, hipStream_t s)
+{
+  hipLaunchKernelGGL(kReduceK<4>, dim3(1), dim3(256), 0, s, partials, n, out, 2);
+}
+
+// ---- copy ------------------------------------------------------------------
+__global__ __launch_bounds__(256) void kCopy(const double2 *__restrict__ src,
+    double2 *__restrict__ dst, uint64_t n2)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride)
+    dst[i] = src[i];
+}
+
+void launchCopy(const double *src, double *dst, uint64_t n, hipStream_t s)
+{
+  const uint64_t n2 = n / 2;
+  hipLaunchKernelGGL(kCopy, dim3(256 * 64), dim3(256), 0, s, (const double2 *)src,
+      (double2 *)dst, n2);
+}
+
+} // namespace lbm
+
+namespace lbm {
+
+// ---- solid-mask windows ------------------------------------------------------
+// one thread per (seg, y, x), x fastest so that the flag reads of a
+// wavefront are contiguous
+__global__ __launch_bounds__(256) void kBuildMask(const int *__restrict__ lat,
+    MaskWin *__restrict__ mask, int nx, int ny, int nz, int segs)
+{
+  const uint64_t t     = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t total = (uint64_t)nx * ny * segs;
+  if (t >= total)
+    return;
+  const int x   = (int)(t % nx);
+  const int y   = (int)((t / nx) % ny);
+  const int seg = (int)(t / ((uint64_t)nx * ny));
+  uint64_t lo = 0, hi = 0;
+  for (int k = 0; k < 66; ++k) {
+    int64_t zz = ((int64_t)seg * 64 - 1 + k) % nz;
+    if (zz < 0)
+      zz += nz;
+    const uint64_t bit = lat[((uint64_t)zz * ny + y) * nx + x] == 0 ? 1ull : 0ull;
+    if (k == 0)
+      hi |= bit;
+    else if (k <= 64)
+      lo |= bit << (k - 1);
+    else
+      hi |= bit << 1;
+  }
+  mask[((uint64_t)x * ny + y) * segs + seg] = MaskWin{ lo, hi };
+}
+
+void launchBuildMask(const int *lattice, MaskWin *mask, int nx, int ny, int nz, int segs,
+    hipStream_t s)
+{
+  const uint64_t total  = (uint64_t)nx * ny * segs;
+  const unsigned blocks = (unsigned)((total + 255) / 256);
+  hipLaunchKernelGGL(kBuildMask, dim3(blocks), dim3(256), 0, s, lattice, mask, nx, ny, nz, segs);
+}
+
+} // namespace lbm

@@ -1,0 +1,183 @@
+// lbm_device.h -- device-side building blocks of the D3Q19 TRT step for
+// gfx950: velocity set, TRT collision, storage conversion, solid-mask
+// windows and the per-variant slot addressing.
+//
+// Data layout in HBM (one lattice, no halo):
+//   cell   = (x*nY + y)*nZ + z           z is the unit-stride axis
+//   SoA    element (d, cell) at d*plane + cell, plane >= nX*nY*nZ (padded)
+//   AoS    element (d, cell) at cell*19 + d
+//   mask   one MaskWin per (row = x*nY + y, seg = z/64): 66 solid bits
+//          w[k] = solid(x, y, (64*seg - 1 + k) mod nZ), k = 0..65,
+//          stored as lo = w[1..64], hi bit0 = w[0], hi bit1 = w[65].
+//          Lane l of the wavefront that owns z = 64*seg + l reads its
+//          z-1, z, z+1 bits as w[l], w[l+1], w[l+2].
+// Periodicity is index wrap-around on every axis; a non-periodic axis has
+// solid boundary layers (as LBM-Bench's geometryCreate builds them,
+// src/lbm.c:104-129), which no fluid cell ever looks past.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+
+namespace lbm {
+
+constexpr int Q = 19;
+// direction ids of LBM-Bench (src/lbm.h:20-38)
+enum Dir { N = 0, S, E, W, NE, SE, NW, SW, T, TN, TE, TW, TS, B, BS, BN, BW, BE, C };
+
+__host__ __device__ constexpr int cx(int d)
+{
+  constexpr int v[Q] = { 0, 0, 1, -1, 1, 1, -1, -1, 0, 0, 1, -1, 0, 0, 0, 0, -1, 1, 0 };
+  return v[d];
+}
+__host__ __device__ constexpr int cy(int d)
+{
+  constexpr int v[Q] = { 1, -1, 0, 0, 1, -1, 1, -1, 0, 1, 0, 0, -1, 0, -1, 1, 0, 0, 0 };
+  return v[d];
+}
+__host__ __device__ constexpr int cz(int d)
+{
+  constexpr int v[Q] = { 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, -1, -1, -1, -1, -1, 0 };
+  return v[d];
+}
+__host__ __device__ constexpr int inv(int d)
+{
+  constexpr int v[Q] = { S, N, W, E, SW, NW, SE, NE, B, BS, BW, BE, BN, T, TN, TS, TE, TW, C };
+  return v[d];
+}
+// weight class: 0 centre (1/3), 1 axis (1/18), 2 diagonal (1/36)
+__host__ __device__ constexpr int wclass(int d)
+{
+  return d == C ? 0 : ((cx(d) != 0) + (cy(d) != 0) + (cz(d) != 0) == 1 ? 1 : 2);
+}
+
+// TRT constants, formed on the host in the order of
+// src/kernel_pull_soa.c:18-28 with Lambda as a parameter.
+template <typename R>
+struct Trt {
+  R we, wo, w0, w1, w2, w1x3, w2x3, w1nh, w2nh;
+};
+
+// TRT collision in place (src/kernel_pull_soa.c:118-198): f holds the 19
+// pre-collision values and receives the post-collision ones, both indexed
+// by direction.  Moments are summed in the reference's order.
+template <typename R>
+__device__ __forceinline__ void collide(R (&f)[Q], const Trt<R> &p)
+{
+  const R ux = f[E] + f[NE] + f[SE] + f[TE] + f[BE] - f[W] - f[NW] - f[SW] - f[TW] - f[BW];
+  const R uy = f[N] + f[NE] + f[NW] + f[TN] + f[BN] - f[S] - f[SE] - f[SW] - f[TS] - f[BS];
+  const R uz = f[T] + f[TE] + f[TW] + f[TN] + f[TS] - f[B] - f[BE] - f[BW] - f[BN] - f[BS];
+  const R rho = f[C] + f[N] + f[E] + f[S] + f[W] + f[NE] + f[SE] + f[SW] + f[NW] + f[T] +
+                f[TN] + f[TE] + f[TS] + f[TW] + f[B] + f[BN] + f[BE] + f[BS] + f[BW];
+  const R D = rho - (ux * ux + uy * uy + uz * uz) * R(3.0) / R(2.0);
+
+  f[C] = f[C] - p.we * (f[C] - p.w0 * D);
+
+  const R w1D = p.w1 * D, w2D = p.w2 * D;
+  auto pair = [&](int i, int j, R ui, R wX3, R wNH, R wD) {
+    const R fi = f[i], fj = f[j];
+    const R even = p.we * (R(0.5) * (fi + fj) - ui * ui * wNH - wD);
+    const R odd  = p.wo * (R(0.5) * (fi - fj) - ui * wX3);
+    f[i] = fi - even - odd;
+    f[j] = fj - even + odd;
+  };
+  pair(N, S, uy, p.w1x3, p.w1nh, w1D);
+  pair(E, W, ux, p.w1x3, p.w1nh, w1D);
+  pair(T, B, uz, p.w1x3, p.w1nh, w1D);
+  pair(NW, SE, -ux + uy, p.w2x3, p.w2nh, w2D);
+  pair(NE, SW, ux + uy, p.w2x3, p.w2nh, w2D);
+  pair(TW, BE, -ux + uz, p.w2x3, p.w2nh, w2D);
+  pair(TE, BW, ux + uz, p.w2x3, p.w2nh, w2D);
+  pair(TS, BN, -uy + uz, p.w2x3, p.w2nh, w2D);
+  pair(TN, BS, uy + uz, p.w2x3, p.w2nh, w2D);
+}
+
+// ---- storage forms ------------------------------------------------------
+// Native storage keeps f itself; the low-precision forms keep
+// (f - w*RhoRef) * scale with scale a power of two, so the conversion to
+// the FP32 compute type is one exact multiply.  The collision is applied
+// to the deviation directly: the incompressible equilibrium is linear in
+// rho, so the w*RhoRef part drops out of every term.
+template <typename S>
+struct Store;
+template <>
+struct Store<double> {
+  using R = double;
+  static constexpr bool scaled = false;
+  __device__ static R load(double v, R) { return v; }
+  __device__ static double store(R v, R) { return v; }
+};
+template <>
+struct Store<float> {
+  using R = float;
+  static constexpr bool scaled = false;
+  __device__ static R load(float v, R) { return v; }
+  __device__ static float store(R v, R) { return v; }
+};
+// FP32 shifted-deviation storage: same bytes as float, scaled
+struct f32dev {
+  float v;
+};
+template <>
+struct Store<f32dev> {
+  using R = float;
+  static constexpr bool scaled = true;
+  __device__ static R load(f32dev v, R inv) { return v.v * inv; }
+  __device__ static f32dev store(R v, R sc) { return f32dev{ v * sc }; }
+};
+template <>
+struct Store<__half> {
+  using R = float;
+  static constexpr bool scaled = true;
+  __device__ static R load(__half v, R inv) { return __half2float(v) * inv; }
+  __device__ static __half store(R v, R sc) { return __float2half_rn(v * sc); }
+};
+template <>
+struct Store<__hip_bfloat16> {
+  using R = float;
+  static constexpr bool scaled = true;
+  __device__ static R load(__hip_bfloat16 v, R inv) { return __bfloat162float(v) * inv; }
+  __device__ static __hip_bfloat16 store(R v, R sc) { return __float2bfloat16(v * sc); }
+};
+
+// ---- solid mask -----------------------------------------------------------
+struct alignas(16) MaskWin {
+  uint64_t lo, hi;
+};
+
+// window shifted so that bit l is the z-1 / z / z+1 neighbour of lane l
+struct RowBits {
+  uint64_t m, c, p; // z-1, z, z+1
+};
+__device__ __forceinline__ RowBits rowBits(MaskWin w)
+{
+  RowBits r;
+  r.c = w.lo;
+  r.m = (w.lo << 1) | (w.hi & 1ull);
+  r.p = (w.lo >> 1) | ((w.hi >> 1) << 63);
+  return r;
+}
+__device__ __forceinline__ bool bitAt(uint64_t word, int lane)
+{
+  return (word >> lane) & 1ull;
+}
+
+// per-lane lookup of an arbitrary cell (used by the tiled, AoS-free
+// generic passes: init, force, reductions)
+__device__ __forceinline__ bool solidAt(const MaskWin *__restrict__ mask, int segs, int nY,
+    int x, int y, int z)
+{
+  const MaskWin w = mask[((size_t)x * nY + y) * segs + (z >> 6)];
+  return (w.lo >> (z & 63)) & 1ull;
+}
+
+__device__ __forceinline__ int wrapInc(int v, int n) { return v + 1 == n ? 0 : v + 1; }
+__device__ __forceinline__ int wrapDec(int v, int n) { return v == 0 ? n - 1 : v - 1; }
+__device__ __forceinline__ int wrapAdd(int v, int c, int n)
+{
+  return c > 0 ? wrapInc(v, n) : (c < 0 ? wrapDec(v, n) : v);
+}
+
+} // namespace lbm

@@ -1,0 +1,377 @@
+/* lbm_host.c -- host-side parts of the lbm.h boundary that do not touch
+ * the GPU directly: the D3Q19 tables and kernel registry, geometry,
+ * LBM-Bench's initial conditions, statistics, verification and the Zou-He
+ * boundary routine.  Everything that reads PDFs goes through the kernel's
+ * GetNode/SetNode (the device mirror of lbm_runtime.hip), exactly as the
+ * reference's lbm.c does.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <strings.h>
+
+#include "lbm.h"
+
+/* velocity set and inverse directions (src/lbm.c:20-29) */
+int D3Q19X[N_D3Q19]   = { 0, 0, 1, -1, 1, 1, -1, -1, 0, 0, 1, -1, 0, 0, 0, 0, -1, 1, 0 };
+int D3Q19Y[N_D3Q19]   = { 1, -1, 0, 0, 1, -1, 1, -1, 0, 1, 0, 0, -1, 0, -1, 1, 0, 0, 0 };
+int D3Q19Z[N_D3Q19]   = { 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, -1, -1, -1, -1, -1, 0 };
+int D3Q19Inv[N_D3Q19] = { D3Q19_S, D3Q19_N, D3Q19_W, D3Q19_E, D3Q19_SW, D3Q19_NW, D3Q19_SE,
+  D3Q19_NE, D3Q19_B, D3Q19_BS, D3Q19_BW, D3Q19_BE, D3Q19_BN, D3Q19_T, D3Q19_TN, D3Q19_TS,
+  D3Q19_TE, D3Q19_TW, D3Q19_C };
+
+/* registry: same names and order as src/lbm.c:33-41; every entry is a GPU
+ * kernel */
+KernelFunctionsType GKernels[] = {
+  { "pull-soa", kernelInitPullSoA, kernelDeinit },
+  { "pull-aos", kernelInitPullAoS, kernelDeinit },
+  { "push-soa", kernelInitPushSoA, kernelDeinit },
+  { "push-aos", kernelInitPushAoS, kernelDeinit },
+  { "blk-push-soa", kernelInitBlkPushSoA, kernelDeinit },
+  { "blk-pull-soa", kernelInitBlkPullSoA, kernelDeinit },
+  { "aa-soa", kernelInitAaSoA, kernelDeinit },
+};
+const int G_N_KERNELS = sizeof(GKernels) / sizeof(GKernels[0]);
+
+void caseDataDefaults(CaseDataType *cd)
+{
+  cd->MagicParam   = 1.0 / 12.0;
+  cd->BodyForce    = 0;
+  cd->Storage      = LBM_STORE_NATIVE;
+  cd->StorageScale = 0.0;
+  cd->RhoRef       = 1.0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* geometry (src/lbm.c:49-187) and the seeded sphere packing                 */
+/* ------------------------------------------------------------------------ */
+static uint64_t GGeometrySeed = 0;
+
+void geometrySetSeed(uint64_t seed) { GGeometrySeed = seed; }
+
+/* splitmix64 finalizer (Steele, Lea & Flood 2014) chained over (seed, a, b);
+ * the same chain seeds the device initialiser */
+static uint64_t mix64(uint64_t x)
+{
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+static double hashU01(uint64_t seed, uint64_t a, uint64_t b)
+{
+  return (double)(mix64(mix64(mix64(seed) ^ a) ^ b) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+static const char *GGeoTypeStr[] = { "box", "channel", "pipe", "blocks", "fluid", "spheres" };
+enum { GEO_BOX, GEO_CHANNEL, GEO_PIPE, GEO_BLOCKS, GEO_FLUID, GEO_SPHERES };
+
+#define SPHERE_SALT 0x5350484552455321ull
+
+static void carveSpheres(LatticeType *lat, const int dims[3], const int per[3], int count,
+    uint64_t seed)
+{
+  for (int k = 0; k < count; ++k) {
+    double c[3];
+    for (int a = 0; a < 3; ++a)
+      c[a] = hashU01(seed ^ SPHERE_SALT, (uint64_t)k, (uint64_t)a) * dims[a];
+    const double r = 4.0 + 8.0 * hashU01(seed ^ SPHERE_SALT, (uint64_t)k, 3);
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = (int)floor(c[a] - r);
+      hi[a] = (int)ceil(c[a] + r);
+    }
+    for (int z = lo[2]; z <= hi[2]; ++z) {
+      int pz = z;
+      if (pz < 0 || pz >= dims[2]) {
+        if (!per[2])
+          continue;
+        pz = ((pz % dims[2]) + dims[2]) % dims[2];
+      }
+      for (int y = lo[1]; y <= hi[1]; ++y) {
+        int py = y;
+        if (py < 0 || py >= dims[1]) {
+          if (!per[1])
+            continue;
+          py = ((py % dims[1]) + dims[1]) % dims[1];
+        }
+        for (int x = lo[0]; x <= hi[0]; ++x) {
+          const double dx = x - c[0], dy = y - c[1], dz = z - c[2];
+          if (dx * dx + dy * dy + dz * dz > r * r)
+            continue;
+          int px = x;
+          if (px < 0 || px >= dims[0]) {
+            if (!per[0])
+              continue;
+            px = ((px % dims[0]) + dims[0]) % dims[0];
+          }
+          lat[((size_t)pz * dims[1] + py) * dims[0] + px] = LAT_CELL_OBSTACLE;
+        }
+      }
+    }
+  }
+}
+
+void geometryCreate(const char *geometryType, int dims[3], int periodic[3], LatticeDescType *ld)
+{
+  int type = -1, blockSize = 8, nSpheres = 0;
+  if (strncasecmp("channel", geometryType, 7) == 0)
+    type = GEO_CHANNEL;
+  else if (strncasecmp("box", geometryType, 3) == 0)
+    type = GEO_BOX;
+  else if (strncasecmp("pipe", geometryType, 4) == 0)
+    type = GEO_PIPE;
+  else if (strncasecmp("blocks", geometryType, 6) == 0) {
+    type = GEO_BLOCKS;
+    if (strlen(geometryType) > 7) {
+      blockSize = atoi(&geometryType[7]);
+      if (blockSize <= 0)
+        blockSize = 8;
+    }
+  } else if (strncasecmp("fluid", geometryType, 5) == 0)
+    type = GEO_FLUID;
+  else if (strncasecmp("spheres", geometryType, 7) == 0) {
+    type     = GEO_SPHERES;
+    nSpheres = strlen(geometryType) > 8 ? atoi(&geometryType[8]) : 0;
+  } else {
+    printf("ERROR: unknown geometry type '%s'.\n", geometryType);
+    exit(1);
+  }
+
+  const size_t ncells = (size_t)dims[0] * dims[1] * dims[2];
+  ld->Dims[0]         = dims[0];
+  ld->Dims[1]         = dims[1];
+  ld->Dims[2]         = dims[2];
+  ld->nCells          = (int)ncells;
+  ld->Name            = GGeoTypeStr[type];
+  LatticeType *lat    = (LatticeType *)malloc(sizeof(LatticeType) * ncells);
+  if (lat == NULL) {
+    fprintf(stderr, "Error: Insufficient memory to fulfill the request\n");
+    exit(EXIT_FAILURE);
+  }
+  ld->Lattice = lat;
+  for (size_t i = 0; i < ncells; ++i)
+    lat[i] = LAT_CELL_FLUID;
+
+  if (type == GEO_CHANNEL || type == GEO_BLOCKS || type == GEO_PIPE || type == GEO_SPHERES)
+    periodic[0] = 1;
+  else if (type == GEO_FLUID)
+    periodic[0] = periodic[1] = periodic[2] = 1;
+
+  const int nx = dims[0], ny = dims[1], nz = dims[2];
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        const int face = (!periodic[0] && (x == 0 || x == nx - 1)) ||
+                         (!periodic[1] && (y == 0 || y == ny - 1)) ||
+                         (!periodic[2] && (z == 0 || z == nz - 1));
+        if (face)
+          lat[((size_t)z * ny + y) * nx + x] = LAT_CELL_OBSTACLE;
+      }
+
+  if (type == GEO_PIPE) {
+    const double cz = nz / 2.0 - 0.5, cy = ny / 2.0 - 0.5;
+    const double dmin = MIN(ny, nz);
+    const double r2   = (dmin / 2 - 1) * (dmin / 2 - 1);
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        if ((z - cz) * (z - cz) + (y - cy) * (y - cy) >= r2)
+          for (int x = 0; x < nx; ++x)
+            lat[((size_t)z * ny + y) * nx + x] = LAT_CELL_OBSTACLE;
+  } else if (type == GEO_BLOCKS) {
+    const int nb = blockSize;
+    for (int z = 0; z < nz; ++z) {
+      if ((z % (2 * nb)) < nb)
+        continue;
+      for (int y = 0; y < ny; ++y) {
+        if ((y % (2 * nb)) < nb)
+          continue;
+        for (int x = 0; x < nx; ++x)
+          if ((x % (2 * nb)) >= nb)
+            lat[((size_t)z * ny + y) * nx + x] = LAT_CELL_OBSTACLE;
+      }
+    }
+  } else if (type == GEO_SPHERES) {
+    carveSpheres(lat, dims, periodic, nSpheres, GGeometrySeed);
+  }
+
+  ld->PeriodicX = periodic[0];
+  ld->PeriodicY = periodic[1];
+  ld->PeriodicZ = periodic[2];
+  ld->nObst = ld->nFluid = ld->nInlet = ld->nOutlet = 0;
+  for (size_t i = 0; i < ncells; ++i) {
+    switch (lat[i]) {
+    case LAT_CELL_OBSTACLE:
+      ld->nObst++;
+      break;
+    case LAT_CELL_FLUID:
+      ld->nFluid++;
+      break;
+    case LAT_CELL_INLET:
+      ld->nInlet++;
+      ld->nFluid++;
+      break;
+    case LAT_CELL_OUTLET:
+      ld->nOutlet++;
+      ld->nFluid++;
+      break;
+    }
+  }
+}
+
+void geometryDestroy(LatticeDescType *ld)
+{
+  free(ld->Lattice);
+  ld->Lattice = NULL;
+}
+
+/* ------------------------------------------------------------------------ */
+/* initial conditions (src/lbm.c:791-896): the incompressible equilibrium     */
+/* scaled by omega, written through SetNode.                                  */
+/* ------------------------------------------------------------------------ */
+static void equilibriumScaled(PdfType dens, PdfType ux, PdfType uy, PdfType uz, PdfType omega,
+    PdfType *pdfs)
+{
+  const PdfType w0 = F(1.0) / F(3.0), w1 = F(1.0) / F(18.0), w2 = F(1.0) / F(36.0);
+  const PdfType ow0 = F(3.0) * w0 * omega, ow1 = F(3.0) * w1 * omega, ow2 = F(3.0) * w2 * omega;
+  const PdfType indep = F(1.0) / F(3.0) * dens - F(0.5) * (ux * ux + uy * uy + uz * uz);
+  /* direction d gets ow_d * (indep + u_d + 1.5 u_d^2), u_d = c_d . u */
+  pdfs[D3Q19_C] = ow0 * indep;
+  for (int d = 0; d < N_D3Q19 - 1; ++d) {
+    const PdfType ud = (PdfType)D3Q19X[d] * ux + (PdfType)D3Q19Y[d] * uy + (PdfType)D3Q19Z[d] * uz;
+    const int axis   = (D3Q19X[d] != 0) + (D3Q19Y[d] != 0) + (D3Q19Z[d] != 0) == 1;
+    pdfs[d]          = (axis ? ow1 : ow2) * (indep + ud + F(1.5) * (ud * ud));
+  }
+}
+
+void kernelSetInitialDensity(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd)
+{
+  const int nX = ld->Dims[0], nY = ld->Dims[1], nZ = ld->Dims[2];
+  PdfType pdfs[N_D3Q19];
+  for (int z = 0; z < nZ; ++z)
+    for (int y = 0; y < nY; ++y)
+      for (int x = 0; x < nX; ++x) {
+        if (ld->Lattice[latticeIndex(ld->Dims, x, y, z)] == LAT_CELL_OBSTACLE)
+          continue;
+        const PdfType dens = cd->RhoIn + (cd->RhoOut - cd->RhoIn) * x / (nX - F(1.0));
+        equilibriumScaled(dens, F(0.0), F(0.0), F(0.0), cd->Omega, pdfs);
+        kd->SetNode(kd, x, y, z, pdfs);
+      }
+}
+
+void kernelSetInitialVelocity(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd)
+{
+  const int nX = ld->Dims[0], nY = ld->Dims[1], nZ = ld->Dims[2];
+  PdfType pdfs[N_D3Q19];
+  for (int z = 0; z < nZ; ++z)
+    for (int y = 0; y < nY; ++y)
+      for (int x = 0; x < nX; ++x) {
+        if (ld->Lattice[latticeIndex(ld->Dims, x, y, z)] != LAT_CELL_FLUID)
+          continue;
+        equilibriumScaled(F(1.0), F(0.0), F(0.0), F(0.0), cd->Omega, pdfs);
+        kd->SetNode(kd, x, y, z, pdfs);
+      }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Zou-He pressure inlet/outlet on x = 0 / x = nX-1 (src/lbm.c:676-785).     */
+/* Never called by the reference's driver; kept for API completeness.         */
+/* ------------------------------------------------------------------------ */
+void kernelComputeBoundaryConditions(KernelDataType *kd, LatticeDescType *ld, CaseDataType *cd)
+{
+  const PdfType third = F(1.0) / F(3.0), fourth = F(1.0) / F(4.0), sixth = F(1.0) / F(6.0);
+  const int nX = kd->Dims[0], nY = kd->Dims[1], nZ = kd->Dims[2];
+  PdfType p[N_D3Q19];
+  /* populations read at the inlet (all but the five entering +x ones) and
+   * at the outlet (all but the five entering -x ones) */
+  static const int inRead[14]  = { D3Q19_C, D3Q19_T, D3Q19_B, D3Q19_S, D3Q19_N, D3Q19_TS,
+    D3Q19_BS, D3Q19_TN, D3Q19_BN, D3Q19_SW, D3Q19_TW, D3Q19_W, D3Q19_BW, D3Q19_NW };
+  static const int outRead[14] = { D3Q19_C, D3Q19_T, D3Q19_B, D3Q19_S, D3Q19_N, D3Q19_TS,
+    D3Q19_BS, D3Q19_TN, D3Q19_BN, D3Q19_NE, D3Q19_BE, D3Q19_E, D3Q19_TE, D3Q19_SE };
+  for (int z = 1; z < nZ - 1; ++z)
+    for (int y = 1; y < nY - 1; ++y) {
+      if (ld->Lattice[latticeIndex(ld->Dims, 0, y, z)] == LAT_CELL_INLET) {
+        for (int k = 0; k < 14; ++k)
+          kd->BoundaryConditionsGetPdf(kd, 0, y, z, inRead[k], &p[inRead[k]]);
+        const PdfType dens = cd->RhoIn;
+        const PdfType ux   = F(1.0) - (p[D3Q19_C] + (p[D3Q19_T] + p[D3Q19_B] + p[D3Q19_S] + p[D3Q19_N]) +
+                                        (p[D3Q19_TS] + p[D3Q19_BS] + p[D3Q19_TN] + p[D3Q19_BN]) +
+                                        F(2.0) * (p[D3Q19_SW] + p[D3Q19_TW] + p[D3Q19_W] +
+                                                     p[D3Q19_BW] + p[D3Q19_NW])) /
+                                       dens;
+        const PdfType indep = sixth * dens * ux;
+        const PdfType ns = fourth * (p[D3Q19_N] - p[D3Q19_S]), tb = fourth * (p[D3Q19_T] - p[D3Q19_B]);
+        kd->BoundaryConditionsSetPdf(kd, 0, y, z, D3Q19_E, p[D3Q19_W] + third * dens * ux);
+        kd->BoundaryConditionsSetPdf(kd, 0, y, z, D3Q19_NE, p[D3Q19_SW] - ns + indep);
+        kd->BoundaryConditionsSetPdf(kd, 0, y, z, D3Q19_SE, p[D3Q19_NW] + ns + indep);
+        kd->BoundaryConditionsSetPdf(kd, 0, y, z, D3Q19_TE, p[D3Q19_BW] - tb + indep);
+        kd->BoundaryConditionsSetPdf(kd, 0, y, z, D3Q19_BE, p[D3Q19_TW] + tb + indep);
+      }
+      if (ld->Lattice[latticeIndex(ld->Dims, nX - 1, y, z)] == LAT_CELL_OUTLET) {
+        const int xo = nX - 1;
+        for (int k = 0; k < 14; ++k)
+          kd->BoundaryConditionsGetPdf(kd, xo, y, z, outRead[k], &p[outRead[k]]);
+        const PdfType dens = cd->RhoOut;
+        const PdfType ux   = F(-1.0) + (p[D3Q19_C] + (p[D3Q19_T] + p[D3Q19_B] + p[D3Q19_S] + p[D3Q19_N]) +
+                                         (p[D3Q19_TS] + p[D3Q19_BS] + p[D3Q19_TN] + p[D3Q19_BN]) +
+                                         F(2.0) * (p[D3Q19_NE] + p[D3Q19_BE] + p[D3Q19_E] +
+                                                      p[D3Q19_TE] + p[D3Q19_SE])) /
+                                        dens;
+        const PdfType indep = sixth * dens * ux;
+        const PdfType ns = fourth * (p[D3Q19_N] - p[D3Q19_S]), tb = fourth * (p[D3Q19_T] - p[D3Q19_B]);
+        kd->BoundaryConditionsSetPdf(kd, xo, y, z, D3Q19_W, p[D3Q19_E] - third * dens * ux);
+        kd->BoundaryConditionsSetPdf(kd, xo, y, z, D3Q19_SW, p[D3Q19_NE] + ns - indep);
+        kd->BoundaryConditionsSetPdf(kd, xo, y, z, D3Q19_NW, p[D3Q19_SE] - ns - indep);
+        kd->BoundaryConditionsSetPdf(kd, xo, y, z, D3Q19_BW, p[D3Q19_TE] + tb - indep);
+        kd->BoundaryConditionsSetPdf(kd, xo, y, z, D3Q19_TW, p[D3Q19_BE] - tb - indep);
+      }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* statistics and verification (src/lbm.c:953-1023)                           */
+/* ------------------------------------------------------------------------ */
+void kernelStatistics(KernelDataType *kd, LatticeDescType *ld, CaseDataType *cd, int iter)
+{
+  if (iter % cd->StatisticsModulus == 0 || iter == cd->MaxIterations)
+    printf("# iter: %4d   avg density: %e\n", iter, (double)kernelDensity(kd, ld));
+}
+
+void kernelVerify(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd, PdfType *errorNorm)
+{
+  const int nX = ld->Dims[0], nY = ld->Dims[1], nZ = ld->Dims[2];
+  const PdfType visc = (F(1.0) / cd->Omega - F(0.5)) / F(3.0);
+  PdfType *prof      = (PdfType *)calloc((size_t)nZ * nY, sizeof(PdfType));
+  PdfType pdfs[N_D3Q19];
+  const int xPos = nX / 2;
+  for (int z = 0; z < nZ; ++z)
+    for (int y = 0; y < nY; ++y) {
+      if (ld->Lattice[latticeIndex(ld->Dims, xPos, y, z)] == LAT_CELL_OBSTACLE)
+        continue;
+      kd->GetNode(kd, xPos, y, z, pdfs);
+      PdfType ux = pdfs[D3Q19_E] + pdfs[D3Q19_NE] + pdfs[D3Q19_SE] + pdfs[D3Q19_TE] +
+                   pdfs[D3Q19_BE] - pdfs[D3Q19_W] - pdfs[D3Q19_NW] - pdfs[D3Q19_SW] -
+                   pdfs[D3Q19_TW] - pdfs[D3Q19_BW];
+      if (cd->BodyForce)
+        ux += F(0.5) * cd->XForce; /* half-step force correction */
+      prof[y * nZ + z] = ux;
+    }
+  const PdfType centre = nY / F(2.0);
+  const PdfType r2max  = (nY / F(2.0) - F(1.0)) * (nY / F(2.0) - F(1.0));
+  const int even       = nY % 2 == 0;
+  const int yc         = (nY - even - 1) / 2;
+  PdfType dev          = F(0.0);
+  for (int z = 0; z < nZ; ++z) {
+    const PdfType r2 = (z - centre + F(0.5)) * (z - centre + F(0.5));
+    const PdfType an = r2 >= r2max ? F(0.0) : cd->XForce * (r2max - r2) / (F(2.0) * visc);
+    const PdfType avg = even ? (prof[yc * nZ + z] + prof[(yc + 1) * nZ + z]) / F(2.0) : prof[yc * nZ + z];
+    if (an != F(0.0)) {
+      const PdfType rel = (PdfType)fabs((double)(an - avg)) / an;
+      dev += rel * rel;
+    }
+  }
+  *errorNorm = (PdfType)sqrt((double)dev);
+  printf("# Kernel validation: L2 error norm of relative error: %e\n", (double)*errorNorm);
+  free(prof);
+}

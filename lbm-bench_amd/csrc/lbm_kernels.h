@@ -1,0 +1,83 @@
+// lbm_kernels.h -- launch interface between the host runtime
+// (lbm_runtime.hip) and the kernel translation units (lbm_step.hip,
+// lbm_tile.hip, lbm_aux.hip).  Plain structs, no templates, so each kernel
+// file instantiates its own templates and the runtime stays small.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lbm_device.h"
+
+namespace lbm {
+
+// one stream-and-collide step
+enum Variant { V_PULL = 0, V_PUSH = 1, V_AA_EVEN = 2, V_AA_ODD = 3 };
+
+// element type of the PDF arrays
+enum StorageKind { ST_F64 = 0, ST_F32 = 1, ST_F32DEV = 2, ST_F16 = 3, ST_BF16 = 4 };
+
+// where logical f_d(x) is stored between steps (LBM-Bench's GetNode,
+// src/lbm.c:193-294): locally (push, AA after an odd step, every variant
+// before its first step), at x - c_d (pull), or AA after an even step.
+enum SlotRule { SR_LOCAL = 0, SR_PULL = 1, SR_AA_EVEN = 2 };
+
+struct Geom {
+  const MaskWin *mask;
+  uint64_t plane; // SoA elements per direction (>= nx*ny*nz)
+  int nx, ny, nz, segs;
+  int aos;
+};
+
+struct StepLaunch {
+  Geom g;
+  int variant, tiled, storage;
+  void *src, *dst; // AA: src == dst
+  double trt[9];   // we, wo, w0, w1, w2, w1x3, w2x3, w1nh, w2nh
+  float trtf[9];   // the same in FP32, formed in FP32
+  double f1, f2;   // body force terms 3*w1*g and 3*w2*g
+  float f1f, f2f;
+  float scale, invScale;
+  int addForce;
+};
+
+void launchStepRow(const StepLaunch &L, hipStream_t s);  // lbm_step.hip
+void launchStepTile(const StepLaunch &L, hipStream_t s); // lbm_tile.hip
+
+// auxiliary passes over every fluid cell (lbm_aux.hip)
+struct AuxLaunch {
+  Geom g;
+  int storage, rule;
+  void *pdf;
+  double rhoRef;       // shifted-deviation reference density
+  float scale, invScale;
+};
+// f_d += 3 w_d c_dx g at the slot holding logical f_d (src/lbm.c:902-922)
+void launchForce(const AuxLaunch &A, double f1, double f2, hipStream_t s);
+// fill logical PDFs: mode 0 zero, 1 w_d*rho, 2 w_d*(rho + (2u-1)*amp)
+void launchInit(const AuxLaunch &A, int mode, uint64_t seed, double rho, double amp,
+    hipStream_t s);
+// per-block FP64 partial sums {rho, jx, jy, jz} of the deviation moments
+// into partials[4*blocks]; optionally the per-cell moments into field
+// (4 planes of nx*ny*nz doubles, NaN on solid cells); returns block count
+int launchMoments(const AuxLaunch &A, double *partials, double *field, hipStream_t s);
+int momentsBlocks(const Geom &g);
+// sum partials[4*n] into out[4] in a fixed order (deterministic)
+void launchReduce4(const double *partials, int n, double *out, hipStream_t s);
+// compare two moment fields: partials of {sum |dj|^2, sum |jB|^2,
+// max |dj|, sum drho^2}; returns block count
+int launchCompareFields(const double *fa, const double *fb, uint64_t ncells, double *partials,
+    hipStream_t s);
+void launchReduceCompare(const double *partials, int n, double *out, hipStream_t s);
+// plain device copy, for the measured bandwidth line and counter scaling
+void launchCopy(const double *src, double *dst, uint64_t n, hipStream_t s);
+// logical PDFs of every cell to / from a float64 [cell][19] buffer
+// (obstacle cells: written as 0 / skipped), converting the storage form
+void launchGather(const AuxLaunch &A, double *out, hipStream_t s);
+void launchScatter(const AuxLaunch &A, const double *in, hipStream_t s);
+// solid-mask windows from LatticeDescType.Lattice flags (x fastest):
+// window (row, seg) bit k = (flag at z = (64*seg - 1 + k) mod nz) == obstacle
+void launchBuildMask(const int *lattice, MaskWin *mask, int nx, int ny, int nz, int segs,
+    hipStream_t s);
+
+} // namespace lbm

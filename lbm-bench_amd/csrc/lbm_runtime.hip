@@ -1,0 +1,807 @@
+// lbm_runtime.hip -- host side of the C-ABI: the kernel registry entries of
+// LBM-Bench's lbm.h (kernelInit<Variant>, kernel<Variant>, kernelDeinit,
+// GetNode/SetNode, kernelAddBodyForce, kernelDensity) on top of device
+// arrays, plus the entry points of lbm_hip.h.
+//
+// Device state per KernelData (struct LbmDevice):
+//   one (AA) or two (push/pull) PDF arrays, hipMalloc'd, padded SoA planes
+//   or AoS; the solid-mask windows; a non-blocking stream of default
+//   priority; two hipEvents; hipGraphs of G consecutive steps; a host mirror
+//   of the array holding the state, allocated on the first GetNode/SetNode.
+//
+// Step loop of kernel<Variant> (the time loop of src/kernel_*.c):
+//   the first step of a run collides the state as it is; every later step
+//   adds the body force of the step before it to the values it loads
+//   (the reference adds it after bounce-back, src/kernel_pull_soa.c:206-209,
+//   and the value it stores is the value the next step reads); one force
+//   pass after the last step completes the state.  AA runs steps in
+//   even/odd pairs, rounding an odd count up (src/kernel_aa_soa.c:36).
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "lbm.h"
+#include "lbm_hip.h"
+#include "lbm_kernels.h"
+
+using namespace lbm;
+
+#define HIP_OK(call)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess) {                                                                \
+      fprintf(stderr, "ERROR: %s failed: %s (%s:%d)\n", #call, hipGetErrorString(e_),      \
+          __FILE__, __LINE__);                                                             \
+      exit(EXIT_FAILURE);                                                                  \
+    }                                                                                      \
+  } while (0)
+
+static void fail(const char *msg)
+{
+  fprintf(stderr, "ERROR: %s\n", msg);
+  exit(EXIT_FAILURE);
+}
+
+#ifdef PRECISION_SP
+static const int kNativeStorage = ST_F32;
+#else
+static const int kNativeStorage = ST_F64;
+#endif
+
+static const double kWeight[Q] = { 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 36, 1.0 / 36,
+  1.0 / 36, 1.0 / 36, 1.0 / 18, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 18, 1.0 / 36,
+  1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 3 };
+
+struct LbmDevice {
+  int prop, layout, tiled, storage;
+  size_t esize;
+  int nx, ny, nz, segs;
+  uint64_t ncells, plane, nelem;
+  uint64_t nfluid;
+  int narrays;
+  void *d[2];
+  MaskWin *dmask;
+  std::vector<MaskWin> hmask;
+  int active;
+  hipStream_t stream;
+  hipEvent_t ev0, ev1;
+  unsigned char *mirror;
+  bool mirrorValid, mirrorDirty;
+  hipGraphExec_t gexec[2];
+  bool gvalid[2];
+  int gsteps;
+  double gkey[6];
+  double rhoRef;
+  float scale, invScale;
+  double *dpart, *dout;
+  int npart;
+};
+
+// ---- small helpers ---------------------------------------------------------
+static int envInt(const char *name, int dflt)
+{
+  const char *v = getenv(name);
+  return v && *v ? atoi(v) : dflt;
+}
+
+static bool solidHost(const LbmDevice *D, int x, int y, int z)
+{
+  const MaskWin &w = D->hmask[((size_t)x * D->ny + y) * D->segs + (z >> 6)];
+  return (w.lo >> (z & 63)) & 1ull;
+}
+
+static int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+static int currentRule(const LbmDevice *D, const KernelDataType *kd)
+{
+  if (D->prop == PROP_PULL)
+    return SR_PULL;
+  if (D->prop == PROP_AA && kd->Iteration >= 0 && kd->Iteration % 2 == 0)
+    return SR_AA_EVEN;
+  return SR_LOCAL;
+}
+
+// element index of logical f_d(x, y, z) (src/lbm.c:193-294, masked form)
+static uint64_t hostSlot(const LbmDevice *D, int rule, int x, int y, int z, int d)
+{
+  auto sl = [&](uint64_t c, int k) -> uint64_t {
+    return D->layout == LAYOUT_AOS ? c * Q + (uint64_t)k : (uint64_t)k * D->plane + c;
+  };
+  const uint64_t cell = ((uint64_t)x * D->ny + y) * D->nz + z;
+  if (rule == SR_LOCAL || d == C)
+    return sl(cell, d);
+  const int xn = wrap(x - cx(d), D->nx), yn = wrap(y - cy(d), D->ny), zn = wrap(z - cz(d), D->nz);
+  const bool bb    = solidHost(D, xn, yn, zn);
+  const uint64_t n = ((uint64_t)xn * D->ny + yn) * D->nz + zn;
+  if (rule == SR_PULL)
+    return bb ? sl(cell, inv(d)) : sl(n, d);
+  return bb ? sl(cell, d) : sl(n, inv(d));
+}
+
+// storage <-> double on the host
+static float halfToFloat(uint16_t h)
+{
+  _Float16 v;
+  memcpy(&v, &h, 2);
+  return (float)v;
+}
+static uint16_t floatToHalf(float f)
+{
+  _Float16 v = (_Float16)f;
+  uint16_t h;
+  memcpy(&h, &v, 2);
+  return h;
+}
+static float bf16ToFloat(uint16_t b)
+{
+  uint32_t u = (uint32_t)b << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+static uint16_t floatToBf16(float f)
+{
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u)
+    return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+static double loadHost(const LbmDevice *D, uint64_t i, int d)
+{
+  const unsigned char *m = D->mirror;
+  switch (D->storage) {
+  case ST_F64:
+    return ((const double *)m)[i];
+  case ST_F32:
+    return ((const float *)m)[i];
+  case ST_F32DEV:
+    return kWeight[d] * D->rhoRef + (double)(((const float *)m)[i] * D->invScale);
+  case ST_F16:
+    return kWeight[d] * D->rhoRef + (double)(halfToFloat(((const uint16_t *)m)[i]) * D->invScale);
+  default:
+    return kWeight[d] * D->rhoRef + (double)(bf16ToFloat(((const uint16_t *)m)[i]) * D->invScale);
+  }
+}
+
+static void storeHost(LbmDevice *D, uint64_t i, int d, double v)
+{
+  unsigned char *m = D->mirror;
+  switch (D->storage) {
+  case ST_F64:
+    ((double *)m)[i] = v;
+    return;
+  case ST_F32:
+    ((float *)m)[i] = (float)v;
+    return;
+  default:
+    break;
+  }
+  const float dev = (float)(v - kWeight[d] * D->rhoRef) * D->scale;
+  if (D->storage == ST_F32DEV)
+    ((float *)m)[i] = dev;
+  else if (D->storage == ST_F16)
+    ((uint16_t *)m)[i] = floatToHalf(dev);
+  else
+    ((uint16_t *)m)[i] = floatToBf16(dev);
+}
+
+// ---- host mirror -----------------------------------------------------------
+static void pointMirror(KernelDataType *kd)
+{
+  LbmDevice *D = kd->Dev;
+  PdfType *m   = (D->mirror && D->storage == kNativeStorage) ? (PdfType *)D->mirror : NULL;
+  kd->Pdfs[0] = kd->Pdfs[1] = NULL;
+  kd->Pdfs[D->active]       = m;
+  kd->PdfsActive            = m;
+}
+
+static void ensureMirror(KernelDataType *kd)
+{
+  LbmDevice *D = kd->Dev;
+  if (!D->mirror) {
+    D->mirror = (unsigned char *)malloc(D->nelem * D->esize);
+    if (!D->mirror)
+      fail("insufficient host memory for the node mirror");
+    D->mirrorValid = false;
+  }
+  if (!D->mirrorValid) {
+    HIP_OK(hipStreamSynchronize(D->stream));
+    HIP_OK(hipMemcpy(D->mirror, D->d[D->active], D->nelem * D->esize, hipMemcpyDeviceToHost));
+    D->mirrorValid = true;
+    D->mirrorDirty = false;
+  }
+  pointMirror(kd);
+}
+
+static void flushMirror(LbmDevice *D)
+{
+  if (D->mirror && D->mirrorDirty) {
+    HIP_OK(hipStreamSynchronize(D->stream));
+    HIP_OK(hipMemcpy(D->d[D->active], D->mirror, D->nelem * D->esize, hipMemcpyHostToDevice));
+    D->mirrorDirty = false;
+  }
+}
+
+static void getNodeImpl(KernelDataType *kd, int x, int y, int z, PdfType *pdfs)
+{
+  ensureMirror(kd);
+  const int rule = currentRule(kd->Dev, kd);
+  for (int d = 0; d < Q; ++d)
+    pdfs[d] = (PdfType)loadHost(kd->Dev, hostSlot(kd->Dev, rule, x, y, z, d), d);
+}
+
+static void setNodeImpl(KernelDataType *kd, int x, int y, int z, PdfType *pdfs)
+{
+  ensureMirror(kd);
+  const int rule = currentRule(kd->Dev, kd);
+  for (int d = 0; d < Q; ++d)
+    storeHost(kd->Dev, hostSlot(kd->Dev, rule, x, y, z, d), d, (double)pdfs[d]);
+  kd->Dev->mirrorDirty = true;
+}
+
+static void bcGetImpl(KernelDataType *kd, int x, int y, int z, int dir, PdfType *pdf)
+{
+  ensureMirror(kd);
+  const int rule = currentRule(kd->Dev, kd);
+  *pdf           = (PdfType)loadHost(kd->Dev, hostSlot(kd->Dev, rule, x, y, z, dir), dir);
+}
+
+static void bcSetImpl(KernelDataType *kd, int x, int y, int z, int dir, PdfType pdf)
+{
+  ensureMirror(kd);
+  const int rule = currentRule(kd->Dev, kd);
+  storeHost(kd->Dev, hostSlot(kd->Dev, rule, x, y, z, dir), dir, (double)pdf);
+  kd->Dev->mirrorDirty = true;
+}
+
+// ---- launches --------------------------------------------------------------
+static Geom geomOf(const LbmDevice *D)
+{
+  Geom g;
+  g.mask  = D->dmask;
+  g.plane = D->plane;
+  g.nx    = D->nx;
+  g.ny    = D->ny;
+  g.nz    = D->nz;
+  g.segs  = D->segs;
+  g.aos   = D->layout == LAYOUT_AOS;
+  return g;
+}
+
+template <typename R>
+static void trtConstants(R omega, R magic, R *c)
+{
+  // src/kernel_pull_soa.c:18-28, with magic as a parameter
+  const R w1 = R(1.0) / R(18.0), w2 = R(1.0) / R(36.0);
+  c[0] = omega;
+  c[1] = R(1.0) / (R(0.5) + magic / (R(1.0) / omega - R(0.5)));
+  c[2] = R(1.0) / R(3.0);
+  c[3] = w1;
+  c[4] = w2;
+  c[5] = w1 * R(3.0);
+  c[6] = w2 * R(3.0);
+  c[7] = w1 * R(9.0) / R(2.0);
+  c[8] = w2 * R(9.0) / R(2.0);
+}
+
+static StepLaunch baseLaunch(const LbmDevice *D, const CaseDataType *cd)
+{
+  StepLaunch L;
+  memset(&L, 0, sizeof(L));
+  L.g       = geomOf(D);
+  L.tiled   = D->tiled;
+  L.storage = D->storage;
+  trtConstants<double>((double)cd->Omega, cd->MagicParam, L.trt);
+  trtConstants<float>((float)cd->Omega, (float)cd->MagicParam, L.trtf);
+  // 3 * w * c_x * g, evaluated as in src/lbm.c:919
+  L.f1       = 3.0 * (1.0 / 18.0) * (double)cd->XForce;
+  L.f2       = 3.0 * (1.0 / 36.0) * (double)cd->XForce;
+  L.f1f      = 3.0f * (1.0f / 18.0f) * (float)cd->XForce;
+  L.f2f      = 3.0f * (1.0f / 36.0f) * (float)cd->XForce;
+  L.scale    = D->scale;
+  L.invScale = D->invScale;
+  return L;
+}
+
+// one step; for two-lattice variants src is D->d[srcIdx]
+static void launchStep(const LbmDevice *D, StepLaunch L, int variant, int srcIdx, int addForce,
+    hipStream_t s)
+{
+  L.variant  = variant;
+  L.addForce = addForce;
+  if (D->prop == PROP_AA) {
+    L.src = L.dst = D->d[0];
+  } else {
+    L.src = D->d[srcIdx];
+    L.dst = D->d[srcIdx ^ 1];
+  }
+  if (D->tiled)
+    launchStepTile(L, s);
+  else
+    launchStepRow(L, s);
+  HIP_OK(hipGetLastError());
+}
+
+static int stepVariant(const LbmDevice *D) { return D->prop == PROP_PULL ? V_PULL : V_PUSH; }
+
+static void destroyGraphs(LbmDevice *D)
+{
+  for (int i = 0; i < 2; ++i)
+    if (D->gvalid[i]) {
+      HIP_OK(hipGraphExecDestroy(D->gexec[i]));
+      D->gvalid[i] = false;
+    }
+}
+
+// hipGraphs of G steps that all add the force of the step before them:
+// AA -> one graph (even, odd, ...); push/pull -> one per source array.
+static void prepareGraphs(LbmDevice *D, const CaseDataType *cd, const StepLaunch &base)
+{
+  const double key[6] = { (double)cd->Omega, cd->MagicParam, (double)cd->XForce,
+    (double)cd->BodyForce, (double)D->gsteps, (double)D->storage };
+  bool same = D->gvalid[0];
+  for (int i = 0; i < 6 && same; ++i)
+    same = key[i] == D->gkey[i];
+  if (same || D->gsteps < 2)
+    return;
+  destroyGraphs(D);
+  memcpy(D->gkey, key, sizeof(key));
+  const int ngraphs = D->prop == PROP_AA ? 1 : 2;
+  for (int gi = 0; gi < ngraphs; ++gi) {
+    hipGraph_t graph;
+    HIP_OK(hipStreamBeginCapture(D->stream, hipStreamCaptureModeThreadLocal));
+    int src = gi;
+    for (int s = 0; s < D->gsteps; ++s) {
+      if (D->prop == PROP_AA)
+        launchStep(D, base, (s & 1) ? V_AA_ODD : V_AA_EVEN, 0, cd->BodyForce, D->stream);
+      else {
+        launchStep(D, base, stepVariant(D), src, cd->BodyForce, D->stream);
+        src ^= 1;
+      }
+    }
+    HIP_OK(hipStreamEndCapture(D->stream, &graph));
+    HIP_OK(hipGraphInstantiate(&D->gexec[gi], graph, nullptr, nullptr, 0));
+    HIP_OK(hipGraphDestroy(graph));
+    D->gvalid[gi] = true;
+  }
+}
+
+static AuxLaunch auxOf(const LbmDevice *D, int rule)
+{
+  AuxLaunch A;
+  A.g        = geomOf(D);
+  A.storage  = D->storage;
+  A.rule     = rule;
+  A.pdf      = D->d[D->active];
+  A.rhoRef   = D->rhoRef;
+  A.scale    = D->scale;
+  A.invScale = D->invScale;
+  return A;
+}
+
+static void runKernel(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd)
+{
+  LbmDevice *D = kd->Dev;
+  flushMirror(D);
+  int n = cd->MaxIterations;
+  if (n < 0)
+    n = 0;
+  if (D->prop == PROP_AA)
+    n = (n + 1) / 2 * 2;
+  const int bf          = cd->BodyForce ? 1 : 0;
+  const StepLaunch base = baseLaunch(D, cd);
+  prepareGraphs(D, cd, base);
+  const int G = D->gvalid[0] ? D->gsteps : 0;
+
+  HIP_OK(hipEventRecord(D->ev0, D->stream));
+  if (n > 0) {
+    int left = n;
+    if (D->prop == PROP_AA) {
+      launchStep(D, base, V_AA_EVEN, 0, 0, D->stream);
+      launchStep(D, base, V_AA_ODD, 0, bf, D->stream);
+      left -= 2;
+      while (G > 0 && left >= G) {
+        HIP_OK(hipGraphLaunch(D->gexec[0], D->stream));
+        left -= G;
+      }
+      while (left > 0) {
+        launchStep(D, base, V_AA_EVEN, 0, bf, D->stream);
+        launchStep(D, base, V_AA_ODD, 0, bf, D->stream);
+        left -= 2;
+      }
+    } else {
+      launchStep(D, base, stepVariant(D), D->active, 0, D->stream);
+      D->active ^= 1;
+      left -= 1;
+      while (G > 0 && left >= G) {
+        HIP_OK(hipGraphLaunch(D->gexec[D->active], D->stream));
+        left -= G; // G is even: the state ends in the same array
+      }
+      while (left > 0) {
+        launchStep(D, base, stepVariant(D), D->active, bf, D->stream);
+        D->active ^= 1;
+        left -= 1;
+      }
+    }
+    if (bf) {
+      const int rule = D->prop == PROP_PULL ? SR_PULL : SR_LOCAL;
+      launchForce(auxOf(D, rule), base.f1, base.f2, D->stream);
+      HIP_OK(hipGetLastError());
+    }
+  }
+  HIP_OK(hipEventRecord(D->ev1, D->stream));
+  HIP_OK(hipEventSynchronize(D->ev1));
+  float ms = 0.0f;
+  HIP_OK(hipEventElapsedTime(&ms, D->ev0, D->ev1));
+  kd->Duration = (double)ms * 1e-3;
+  if (D->prop == PROP_AA && n > 0)
+    kd->Iteration = n - 1;
+  D->mirrorValid = false;
+  pointMirror(kd);
+}
+
+// ---- init / deinit ------------------------------------------------------------
+static int storageKind(const CaseDataType *cd)
+{
+  switch (cd->Storage) {
+  case LBM_STORE_NATIVE:
+    return kNativeStorage;
+  case LBM_STORE_FP32:
+    return ST_F32DEV;
+  case LBM_STORE_FP16:
+    return ST_F16;
+  case LBM_STORE_BF16:
+    return ST_BF16;
+  default:
+    fail("unknown storage form");
+  }
+  return kNativeStorage;
+}
+
+static size_t storageBytes(int st)
+{
+  return st == ST_F64 ? 8 : (st == ST_F32 || st == ST_F32DEV ? 4 : 2);
+}
+
+static void checkGeometry(const LatticeDescType *ld)
+{
+  const int nx = ld->Dims[0], ny = ld->Dims[1], nz = ld->Dims[2];
+  const int per[3] = { ld->PeriodicX, ld->PeriodicY, ld->PeriodicZ };
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        const bool face = (!per[0] && (x == 0 || x == nx - 1)) ||
+                          (!per[1] && (y == 0 || y == ny - 1)) ||
+                          (!per[2] && (z == 0 || z == nz - 1));
+        if (face && ld->Lattice[((size_t)z * ny + y) * nx + x] != LAT_CELL_OBSTACLE)
+          fail("fluid cell on a non-periodic boundary face; the boundary layers of "
+               "non-periodic axes must be obstacles");
+      }
+}
+
+static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataType *cd,
+    int prop, int layout, int tiled, const char *name,
+    void (*kernel)(LatticeDescType *, KernelDataType *, CaseDataType *))
+{
+  checkGeometry(ld);
+  KernelDataType *kd = (KernelDataType *)calloc(1, sizeof(KernelDataType));
+  LbmDevice *D       = new LbmDevice();
+  if (!kd)
+    fail("insufficient host memory");
+  *kdOut   = kd;
+  kd->Dev  = D;
+  kd->Name = name;
+  for (int a = 0; a < 3; ++a) {
+    kd->Dims[a]       = ld->Dims[a];
+    kd->GlobalDims[a] = ld->Dims[a];
+    kd->Offsets[a]    = 0;
+    kd->Blk[a]        = tiled ? 8 : ld->Dims[a];
+  }
+  kd->Iteration = -1;
+  kd->Kernel    = kernel;
+  kd->GetNode   = getNodeImpl;
+  kd->SetNode   = setNodeImpl;
+  kd->BoundaryConditionsGetPdf = bcGetImpl;
+  kd->BoundaryConditionsSetPdf = bcSetImpl;
+
+  D->prop    = prop;
+  D->layout  = layout;
+  D->tiled   = tiled;
+  D->storage = storageKind(cd);
+  D->esize   = storageBytes(D->storage);
+  D->nx      = ld->Dims[0];
+  D->ny      = ld->Dims[1];
+  D->nz      = ld->Dims[2];
+  D->segs    = (D->nz + 63) / 64;
+  D->ncells  = (uint64_t)D->nx * D->ny * D->nz;
+  D->nfluid  = (uint64_t)ld->nFluid;
+  // SoA planes padded to a multiple of 64 elements plus a stagger, so the
+  // 19 streams of a wavefront do not start at the same DRAM channel offset
+  const uint64_t pad = (uint64_t)envInt("LBM_PLANE_PAD", 64);
+  D->plane           = layout == LAYOUT_SOA ? ((D->ncells + 63) / 64) * 64 + pad : D->ncells;
+  D->nelem           = layout == LAYOUT_SOA ? D->plane * Q : D->ncells * Q;
+  D->narrays         = prop == PROP_AA ? 1 : 2;
+  D->rhoRef          = (double)cd->RhoRef;
+  double sc          = cd->StorageScale;
+  if (!(sc > 0.0))
+    sc = D->storage == ST_F16 ? 4096.0 : 1.0;
+  if (D->storage == kNativeStorage)
+    sc = 1.0;
+  int ex;
+  if (frexp(sc, &ex) != 0.5)
+    fail("StorageScale must be a power of two");
+  D->scale    = (float)sc;
+  D->invScale = (float)(1.0 / sc);
+  if (D->storage == kNativeStorage)
+    D->rhoRef = 0.0;
+  D->gsteps  = envInt("LBM_GRAPH_STEPS", 16) & ~1;
+  kd->LoopBalance = 2.0 * Q * (double)D->esize;
+
+  HIP_OK(hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking));
+  HIP_OK(hipEventCreate(&D->ev0));
+  HIP_OK(hipEventCreate(&D->ev1));
+
+  const size_t bytes = D->nelem * D->esize;
+  printf("# allocating data for %llu LB nodes on the GPU (%llu bytes = %.1f MiB for %s)\n",
+      (unsigned long long)D->ncells, (unsigned long long)(bytes * D->narrays),
+      (double)(bytes * D->narrays) / 1024.0 / 1024.0,
+      D->narrays == 1 ? "single lattice" : "both lattices");
+  for (int i = 0; i < D->narrays; ++i) {
+    HIP_OK(hipMalloc(&D->d[i], bytes));
+    HIP_OK(hipMemsetAsync(D->d[i], 0, bytes, D->stream));
+  }
+
+  // mask windows, built on the device from the flag array
+  const size_t nwin = (size_t)D->nx * D->ny * D->segs;
+  HIP_OK(hipMalloc(&D->dmask, nwin * sizeof(MaskWin)));
+  int *dlat = nullptr;
+  HIP_OK(hipMalloc(&dlat, D->ncells * sizeof(int)));
+  HIP_OK(hipMemcpyAsync(dlat, ld->Lattice, D->ncells * sizeof(int), hipMemcpyHostToDevice,
+      D->stream));
+  launchBuildMask(dlat, D->dmask, D->nx, D->ny, D->nz, D->segs, D->stream);
+  HIP_OK(hipGetLastError());
+  D->hmask.resize(nwin);
+  HIP_OK(hipMemcpyAsync(D->hmask.data(), D->dmask, nwin * sizeof(MaskWin),
+      hipMemcpyDeviceToHost, D->stream));
+  HIP_OK(hipStreamSynchronize(D->stream));
+  HIP_OK(hipFree(dlat));
+
+  D->npart = momentsBlocks(geomOf(D));
+  HIP_OK(hipMalloc(&D->dpart, sizeof(double) * 4 * (size_t)D->npart));
+  HIP_OK(hipMalloc(&D->dout, sizeof(double) * 4));
+  pointMirror(kd);
+}
+
+extern "C" {
+
+void kernelDeinit(LatticeDescType *ld, KernelDataType **kdp)
+{
+  if (kdp == NULL || *kdp == NULL)
+    return;
+  KernelDataType *kd = *kdp;
+  LbmDevice *D       = kd->Dev;
+  if (D) {
+    HIP_OK(hipStreamSynchronize(D->stream));
+    destroyGraphs(D);
+    for (int i = 0; i < D->narrays; ++i)
+      HIP_OK(hipFree(D->d[i]));
+    HIP_OK(hipFree(D->dmask));
+    HIP_OK(hipFree(D->dpart));
+    HIP_OK(hipFree(D->dout));
+    HIP_OK(hipEventDestroy(D->ev0));
+    HIP_OK(hipEventDestroy(D->ev1));
+    HIP_OK(hipStreamDestroy(D->stream));
+    free(D->mirror);
+    delete D;
+  }
+  free(kd);
+  *kdp = NULL;
+}
+
+void kernelPushSoA(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd) { runKernel(ld, kd, cd); }
+void kernelPushAoS(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd) { runKernel(ld, kd, cd); }
+void kernelPullSoA(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd) { runKernel(ld, kd, cd); }
+void kernelPullAoS(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd) { runKernel(ld, kd, cd); }
+void kernelBlkPushSoA(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd) { runKernel(ld, kd, cd); }
+void kernelBlkPullSoA(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd) { runKernel(ld, kd, cd); }
+void kernelAaSoA(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd) { runKernel(ld, kd, cd); }
+
+void kernelInitPushSoA(LatticeDescType *ld, KernelDataType **kd, CaseDataType *cd)
+{
+  initInternal(ld, kd, cd, PROP_PUSH, LAYOUT_SOA, 0, "push-soa", kernelPushSoA);
+}
+void kernelInitPushAoS(LatticeDescType *ld, KernelDataType **kd, CaseDataType *cd)
+{
+  initInternal(ld, kd, cd, PROP_PUSH, LAYOUT_AOS, 0, "push-aos", kernelPushAoS);
+}
+void kernelInitPullSoA(LatticeDescType *ld, KernelDataType **kd, CaseDataType *cd)
+{
+  initInternal(ld, kd, cd, PROP_PULL, LAYOUT_SOA, 0, "pull-soa", kernelPullSoA);
+}
+void kernelInitPullAoS(LatticeDescType *ld, KernelDataType **kd, CaseDataType *cd)
+{
+  initInternal(ld, kd, cd, PROP_PULL, LAYOUT_AOS, 0, "pull-aos", kernelPullAoS);
+}
+void kernelInitBlkPushSoA(LatticeDescType *ld, KernelDataType **kd, CaseDataType *cd)
+{
+  initInternal(ld, kd, cd, PROP_PUSH, LAYOUT_SOA, 1, "blk-push-soa", kernelBlkPushSoA);
+  printf("# blocking x: %3d y: %3d z: %3d\n", 8, 8, 8);
+}
+void kernelInitBlkPullSoA(LatticeDescType *ld, KernelDataType **kd, CaseDataType *cd)
+{
+  initInternal(ld, kd, cd, PROP_PULL, LAYOUT_SOA, 1, "blk-pull-soa", kernelBlkPullSoA);
+  printf("# blocking x: %3d y: %3d z: %3d\n", 8, 8, 8);
+}
+void kernelInitAaSoA(LatticeDescType *ld, KernelDataType **kd, CaseDataType *cd)
+{
+  initInternal(ld, kd, cd, PROP_AA, LAYOUT_SOA, 0, "aa-soa", kernelAaSoA);
+}
+
+void kernelAddBodyForce(KernelDataType *kd, LatticeDescType *ld, CaseDataType *cd)
+{
+  LbmDevice *D = kd->Dev;
+  flushMirror(D);
+  const StepLaunch base = baseLaunch(D, cd);
+  launchForce(auxOf(D, currentRule(D, kd)), base.f1, base.f2, D->stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(D->stream));
+  D->mirrorValid = false;
+}
+
+void lbmMassMomentum(LatticeDescType *ld, KernelDataType *kd, double out[4])
+{
+  LbmDevice *D = kd->Dev;
+  flushMirror(D);
+  const int nb = launchMoments(auxOf(D, currentRule(D, kd)), D->dpart, nullptr, D->stream);
+  HIP_OK(hipGetLastError());
+  launchReduce4(D->dpart, nb, D->dout, D->stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(out, D->dout, 4 * sizeof(double), hipMemcpyDeviceToHost, D->stream));
+  HIP_OK(hipStreamSynchronize(D->stream));
+  // the deviation forms add back sum_i w_i RhoRef = RhoRef per fluid cell
+  out[0] += (double)D->nfluid * D->rhoRef;
+}
+
+PdfType kernelDensity(KernelDataType *kd, LatticeDescType *ld)
+{
+  double m[4];
+  lbmMassMomentum(ld, kd, m);
+  return (PdfType)(m[0] / (double)ld->nFluid);
+}
+
+void lbmCompareMomentum(LatticeDescType *ld, KernelDataType *a, KernelDataType *b, double out[4])
+{
+  LbmDevice *A = a->Dev, *B = b->Dev;
+  if (A->ncells != B->ncells)
+    fail("lbmCompareMomentum: lattices differ in size");
+  flushMirror(A);
+  flushMirror(B);
+  HIP_OK(hipStreamSynchronize(B->stream));
+  double *fa = nullptr, *fb = nullptr;
+  HIP_OK(hipMalloc(&fa, sizeof(double) * 4 * A->ncells));
+  HIP_OK(hipMalloc(&fb, sizeof(double) * 4 * A->ncells));
+  launchMoments(auxOf(A, currentRule(A, a)), A->dpart, fa, A->stream);
+  HIP_OK(hipGetLastError());
+  launchMoments(auxOf(B, currentRule(B, b)), A->dpart, fb, A->stream);
+  HIP_OK(hipGetLastError());
+  // A->dpart holds momentsBlocks() partials, the same block count as compare
+  const int nb = launchCompareFields(fa, fb, A->ncells, A->dpart, A->stream);
+  HIP_OK(hipGetLastError());
+  launchReduceCompare(A->dpart, nb, A->dout, A->stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(out, A->dout, 4 * sizeof(double), hipMemcpyDeviceToHost, A->stream));
+  HIP_OK(hipStreamSynchronize(A->stream));
+  HIP_OK(hipFree(fa));
+  HIP_OK(hipFree(fb));
+}
+
+void lbmGetNodes(LatticeDescType *ld, KernelDataType *kd, double *out)
+{
+  LbmDevice *D = kd->Dev;
+  flushMirror(D);
+  const size_t bytes = sizeof(double) * Q * D->ncells;
+  double *buf        = nullptr;
+  HIP_OK(hipMalloc(&buf, bytes));
+  launchGather(auxOf(D, currentRule(D, kd)), buf, D->stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(D->stream));
+  HIP_OK(hipMemcpy(out, buf, bytes, hipMemcpyDeviceToHost));
+  HIP_OK(hipFree(buf));
+}
+
+void lbmSetNodes(LatticeDescType *ld, KernelDataType *kd, const double *in)
+{
+  LbmDevice *D = kd->Dev;
+  flushMirror(D);
+  const size_t bytes = sizeof(double) * Q * D->ncells;
+  double *buf        = nullptr;
+  HIP_OK(hipMalloc(&buf, bytes));
+  HIP_OK(hipMemcpy(buf, in, bytes, hipMemcpyHostToDevice));
+  launchScatter(auxOf(D, currentRule(D, kd)), buf, D->stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(D->stream));
+  HIP_OK(hipFree(buf));
+  D->mirrorValid = false;
+}
+
+void lbmInitPdfs(LatticeDescType *ld, KernelDataType *kd, int mode, uint64_t seed, double rho,
+    double amp)
+{
+  LbmDevice *D = kd->Dev;
+  if (mode < LBM_INIT_ZERO || mode > LBM_INIT_RANDOM)
+    fail("lbmInitPdfs: unknown mode");
+  launchInit(auxOf(D, currentRule(D, kd)), mode, seed, rho, amp, D->stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(D->stream));
+  D->mirrorValid = false;
+  D->mirrorDirty = false;
+}
+
+double lbmRunSteps(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd, int steps)
+{
+  const int keep    = cd->MaxIterations;
+  cd->MaxIterations = steps;
+  kd->Kernel(ld, kd, cd);
+  cd->MaxIterations = keep;
+  return kd->Duration;
+}
+
+size_t lbmDeviceBytes(KernelDataType *kd)
+{
+  return (size_t)kd->Dev->narrays * kd->Dev->nelem * kd->Dev->esize;
+}
+
+double lbmCopyBandwidth(size_t bytes, int reps)
+{
+  double *a = nullptr, *b = nullptr;
+  const uint64_t n = bytes / sizeof(double);
+  hipStream_t s;
+  hipEvent_t e0, e1;
+  HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  HIP_OK(hipMalloc(&a, n * sizeof(double)));
+  HIP_OK(hipMalloc(&b, n * sizeof(double)));
+  HIP_OK(hipMemsetAsync(a, 0, n * sizeof(double), s));
+  launchCopy(a, b, n, s);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r)
+    launchCopy(a, b, n, s);
+  HIP_OK(hipEventRecord(e1, s));
+  HIP_OK(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+  HIP_OK(hipFree(a));
+  HIP_OK(hipFree(b));
+  HIP_OK(hipEventDestroy(e0));
+  HIP_OK(hipEventDestroy(e1));
+  HIP_OK(hipStreamDestroy(s));
+  return 2.0 * (double)(n / 2 * 2) * sizeof(double) * reps / ((double)ms * 1e-3) / 1e9;
+}
+
+int lbmDeviceSynchronize(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : 1; }
+
+int lbmDeviceInfo(char *name, int len, int *computeUnits, size_t *globalMem)
+{
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+    return 1;
+  if (name && len > 0) {
+    snprintf(name, (size_t)len, "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  if (computeUnits)
+    *computeUnits = prop.multiProcessorCount;
+  if (globalMem)
+    *globalMem = prop.totalGlobalMem;
+  return 0;
+}
+
+} // extern "C"

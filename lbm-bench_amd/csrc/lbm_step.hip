@@ -1,0 +1,170 @@
+// lbm_step.hip -- the stream-and-collide step, one thread per cell along
+// the unit-stride z axis.
+//
+// Mapping: a wavefront owns 64 consecutive z of one (x, y) row
+// (wave w -> row w / segs, segment w % segs); 4 wavefronts per workgroup.
+// Row, segment and the nine neighbouring rows are wavefront-uniform, so the
+// nine mask windows of the 3x3 row neighbourhood come in as scalar loads
+// and each lane picks its z-1 / z / z+1 solid bits by shifting them.
+//
+// Variants (LBM-Bench src/kernel_*.c), all with half-way bounce-back folded
+// into the addressing through the mask instead of the reference's
+// post-pass copy lists (src/lbm.c:465-568):
+//   pull    f_d = solid(x-c_d) ? src[x, inv d] : src[x-c_d, d]; dst[x, d] = f*_d
+//   push    f_d = src[x, d]; solid(x+c_d) ? dst[x, inv d] : dst[x+c_d, d] = f*_d
+//   AA even f_d = a[x, d];   a[x, inv d] = f*_d            (in place)
+//   AA odd  f_d = solid(x-c_d) ? a[x, d] : a[x-c_d, inv d]
+//           solid(x+c_d) ? a[x, inv d] : a[x+c_d, d] = f*_d   (in place)
+// In the AA odd step the slot (y, k) is read and written only by cell
+// y - c_k, or, when that cell is solid, by nobody else than y itself
+// through the bounce-back branch, so cells never race.  Solid cells are
+// skipped entirely.
+#include "lbm_impl.h"
+
+namespace lbm {
+
+template <int V, bool AOS, typename S>
+__global__ __launch_bounds__(256) void kStepRow(S *a, S *b, const MaskWin *__restrict__ mask,
+    Params<S> p)
+{
+  using R = typename Store<S>::R;
+  using St = Store<S>;
+
+  const int wv   = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4u + (threadIdx.x >> 6)));
+  const int lane = (int)(threadIdx.x & 63u);
+  const int row  = wv / p.segs;
+  if (row >= p.nrows)
+    return;
+  const int seg = wv - row * p.segs;
+  const int z   = seg * 64 + lane;
+  if (z >= p.nz)
+    return;
+  const int x = row / p.ny, y = row - x * p.ny;
+
+  const MaskWin own = mask[(uint64_t)row * p.segs + seg];
+  if (bitAt(own.lo, lane))
+    return;
+
+  const int xs[3] = { wrapDec(x, p.nx), x, wrapInc(x, p.nx) };
+  const int ys[3] = { wrapDec(y, p.ny), y, wrapInc(y, p.ny) };
+  const int zs[3] = { wrapDec(z, p.nz), z, wrapInc(z, p.nz) };
+  const uint64_t cell = (uint64_t)row * p.nz + z;
+
+  // row index of the neighbour row (x+ox, y+oy)
+  auto rowOf = [&](int ox, int oy) -> uint64_t {
+    return (uint64_t)xs[ox + 1] * p.ny + ys[oy + 1];
+  };
+  auto cellOf = [&](int ox, int oy, int oz) -> uint64_t {
+    return rowOf(ox, oy) * p.nz + zs[oz + 1];
+  };
+  auto solidOf = [&](int ox, int oy, int oz) -> bool {
+    const MaskWin w = mask[rowOf(ox, oy) * p.segs + seg];
+    const RowBits rb = rowBits(w);
+    return bitAt(oz < 0 ? rb.m : (oz > 0 ? rb.p : rb.c), lane);
+  };
+
+  R f[Q];
+  if constexpr (V == V_PULL || V == V_AA_ODD) {
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+      uint64_t i;
+      if (d == C) {
+        i = slot<AOS>(cell, C, p.plane);
+      } else {
+        const bool bb = solidOf(-cx(d), -cy(d), -cz(d));
+        if (V == V_PULL)
+          i = bb ? slot<AOS>(cell, inv(d), p.plane) : slot<AOS>(cellOf(-cx(d), -cy(d), -cz(d)), d, p.plane);
+        else
+          i = bb ? slot<AOS>(cell, d, p.plane) : slot<AOS>(cellOf(-cx(d), -cy(d), -cz(d)), inv(d), p.plane);
+      }
+      f[d] = St::load(a[i], p.inv);
+    }
+  } else {
+#pragma unroll
+    for (int d = 0; d < Q; ++d)
+      f[d] = St::load(a[slot<AOS>(cell, d, p.plane)], p.inv);
+  }
+
+  if (p.addForce)
+    addForce(f, p.f1, p.f2);
+
+  collide(f, p.trt);
+
+  if constexpr (V == V_PULL) {
+#pragma unroll
+    for (int d = 0; d < Q; ++d)
+      b[slot<AOS>(cell, d, p.plane)] = St::store(f[d], p.scale);
+  } else if constexpr (V == V_AA_EVEN) {
+#pragma unroll
+    for (int d = 0; d < Q; ++d)
+      a[slot<AOS>(cell, inv(d), p.plane)] = St::store(f[d], p.scale);
+  } else {
+    S *o = V == V_PUSH ? b : a;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+      uint64_t i;
+      if (d == C) {
+        i = slot<AOS>(cell, C, p.plane);
+      } else {
+        const bool bb = solidOf(cx(d), cy(d), cz(d));
+        i = bb ? slot<AOS>(cell, inv(d), p.plane) : slot<AOS>(cellOf(cx(d), cy(d), cz(d)), d, p.plane);
+      }
+      o[i] = St::store(f[d], p.scale);
+    }
+  }
+}
+
+template <int V, bool AOS, typename S>
+static void launchT(const StepLaunch &L, hipStream_t s)
+{
+  Params<S> p       = makeParams<S>(L);
+  const uint64_t wv = (uint64_t)p.nrows * p.segs;
+  const unsigned blocks = (unsigned)((wv + 3) / 4);
+  hipLaunchKernelGGL((kStepRow<V, AOS, S>), dim3(blocks), dim3(256), 0, s, (S *)L.src,
+      (S *)L.dst, L.g.mask, p);
+}
+
+template <typename S>
+static void launchS(const StepLaunch &L, hipStream_t s)
+{
+  if (L.g.aos) {
+    switch (L.variant) {
+    case V_PULL:
+      return launchT<V_PULL, true, S>(L, s);
+    case V_PUSH:
+      return launchT<V_PUSH, true, S>(L, s);
+    case V_AA_EVEN:
+      return launchT<V_AA_EVEN, true, S>(L, s);
+    default:
+      return launchT<V_AA_ODD, true, S>(L, s);
+    }
+  }
+  switch (L.variant) {
+  case V_PULL:
+    return launchT<V_PULL, false, S>(L, s);
+  case V_PUSH:
+    return launchT<V_PUSH, false, S>(L, s);
+  case V_AA_EVEN:
+    return launchT<V_AA_EVEN, false, S>(L, s);
+  default:
+    return launchT<V_AA_ODD, false, S>(L, s);
+  }
+}
+
+void launchStepRow(const StepLaunch &L, hipStream_t s)
+{
+  switch (L.storage) {
+  case ST_F64:
+    return launchS<double>(L, s);
+  case ST_F32:
+    return launchS<float>(L, s);
+  case ST_F32DEV:
+    return launchS<f32dev>(L, s);
+  case ST_F16:
+    return launchS<__half>(L, s);
+  default:
+    return launchS<__hip_bfloat16>(L, s);
+  }
+}
+
+} // namespace lbm

@@ -1,0 +1,51 @@
+/* cli.h -- command line of the GPU benchmark driver.
+ *
+ * Same flags and meaning as LBM-Bench's src/cli.h:10-23 (d g i o f k l V t
+ * x y z h; -t is accepted by getopt and then rejected, as there), plus:
+ *   -m V   TRT magic parameter Lambda (reference: fixed 1/12)
+ *   -B     add the body force every step (reference: -DVERIFICATION builds)
+ *   -I T   initial state: zero (reference benchmark start), rest, random
+ *   -s N   seed of the random initial state and of spheres-N geometries
+ *   -a V   amplitude of the random initial state (default 1e-3)
+ *   -p T   PDF storage: native, fp32, fp16, bf16 (shifted deviation)
+ *   -S V   storage scale (power of two; default per form)
+ */
+#ifndef LBM_AMD_CLI_H
+#define LBM_AMD_CLI_H
+
+#include <stdint.h>
+
+#include "lbm.h"
+
+#define BASE_ARGS "d:g:i:o:f:k:lVt:xyzhm:BI:s:a:p:S:"
+
+#define HELPTEXT                                                                         \
+  "Usage: lbmbench [options]\n"                                                          \
+  "  -d XxYxZ             Geometry dimensions\n"                                         \
+  "  -g TYPE              box|channel|pipe|blocks-N|fluid|spheres-N\n"                   \
+  "  -i N                 Number of iterations\n"                                        \
+  "  -o V                 Relaxation parameter (omega)\n"                                \
+  "  -f V                 X-direction body force\n"                                      \
+  "  -k NAME              Kernel to use\n"                                               \
+  "  -l                   List available kernels\n"                                      \
+  "  -V                   Run verification\n"                                            \
+  "  -x/-y/-z             Enable periodic BC in x/y/z\n"                                 \
+  "  -m V                 TRT magic parameter (default 1/12)\n"                          \
+  "  -B                   Apply the body force every step\n"                             \
+  "  -I TYPE              Initial state: zero|rest|random (default zero)\n"              \
+  "  -s N                 Seed for random initial state and spheres\n"                   \
+  "  -a V                 Amplitude of the random initial state\n"                       \
+  "  -p TYPE              PDF storage: native|fp32|fp16|bf16\n"                          \
+  "  -S V                 Storage scale (power of two)\n"                                \
+  "  -h                   Print this help\n"
+
+typedef struct {
+  int initMode; /* LBM_INIT_* */
+  uint64_t seed;
+  double amplitude;
+} RunOptions;
+
+void parseArguments(int argc, char **argv, int dims[3], const char **geometryType,
+    char **kernelToUse, int periodic[3], int *verify, CaseDataType *cd, RunOptions *ro);
+
+#endif

@@ -1,0 +1,275 @@
+"""ctypes binding of the GPU library (liblbmhip-dp.so / liblbmhip-sp.so).
+
+Mirrors the structs of include/lbm.h (LBM-Bench's src/lbm.h:54-103 with
+the appended fields) and wraps the kernel registry the way LBM-Bench's
+driver uses it (src/main.c:93-152): geometryCreate -> GKernels[k].Init ->
+SetNode / device init -> Kernel -> GetNode / statistics -> Deinit.
+
+There is no fallback: if the library is not built, loading raises.
+Logical PDF arrays are float64 numpy arrays (nX, nY, nZ, 19).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+KERNELS = ("pull-soa", "pull-aos", "push-soa", "push-aos",
+           "blk-push-soa", "blk-pull-soa", "aa-soa")
+STORAGE = {"native": 0, "fp64": 0, "fp32": 1, "fp16": 2, "bf16": 3}
+INIT = {"zero": 0, "rest": 1, "random": 2}
+
+c_int, c_double, c_char_p, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_char_p, ctypes.c_void_p
+I3 = c_int * 3
+
+
+def library_path(precision="dp"):
+    return os.path.join(HERE, f"liblbmhip-{precision}.so")
+
+
+def driver_path(precision="dp"):
+    return os.path.join(HERE, f"lbmbench-HIP-{precision}")
+
+
+def _structs(pdf):
+    class LatticeDesc(ctypes.Structure):
+        _fields_ = [("Dims", I3), ("Lattice", ctypes.POINTER(c_int)), ("nCells", c_int),
+                    ("nFluid", c_int), ("nObst", c_int), ("nInlet", c_int), ("nOutlet", c_int),
+                    ("PeriodicX", c_int), ("PeriodicY", c_int), ("PeriodicZ", c_int),
+                    ("Name", c_char_p)]
+
+    class CaseData(ctypes.Structure):
+        _fields_ = [("Omega", pdf), ("RhoIn", pdf), ("RhoOut", pdf), ("XForce", pdf),
+                    ("MaxIterations", c_int), ("StatisticsModulus", c_int),
+                    ("MagicParam", c_double), ("BodyForce", c_int), ("Storage", c_int),
+                    ("StorageScale", c_double), ("RhoRef", c_double)]
+
+    class KernelData(ctypes.Structure):
+        pass
+
+    KDP = ctypes.POINTER(KernelData)
+    LDP = ctypes.POINTER(LatticeDesc)
+    CDP = ctypes.POINTER(CaseData)
+    node_fn = ctypes.CFUNCTYPE(None, KDP, c_int, c_int, c_int, ctypes.POINTER(pdf))
+    bcget_fn = ctypes.CFUNCTYPE(None, KDP, c_int, c_int, c_int, c_int, ctypes.POINTER(pdf))
+    bcset_fn = ctypes.CFUNCTYPE(None, KDP, c_int, c_int, c_int, c_int, pdf)
+    kern_fn = ctypes.CFUNCTYPE(None, LDP, KDP, CDP)
+    KernelData._fields_ = [
+        ("Pdfs", ctypes.POINTER(pdf) * 2), ("PdfsActive", ctypes.POINTER(pdf)),
+        ("Dims", I3), ("GlobalDims", I3), ("Offsets", I3),
+        ("BounceBackPdfsSrc", ctypes.POINTER(c_int)), ("BounceBackPdfsDst", ctypes.POINTER(c_int)),
+        ("nBounceBackPdfs", c_int), ("Blk", I3), ("Iteration", c_int),
+        ("LoopBalance", c_double), ("Duration", c_double),
+        ("GetNode", node_fn), ("SetNode", node_fn),
+        ("BoundaryConditionsGetPdf", bcget_fn), ("BoundaryConditionsSetPdf", bcset_fn),
+        ("Kernel", kern_fn), ("Dev", c_void_p), ("Name", c_char_p)]
+    init_fn = ctypes.CFUNCTYPE(None, LDP, ctypes.POINTER(KDP), CDP)
+    deinit_fn = ctypes.CFUNCTYPE(None, LDP, ctypes.POINTER(KDP))
+
+    class KernelFunctions(ctypes.Structure):
+        _fields_ = [("Name", c_char_p), ("Init", init_fn), ("Deinit", deinit_fn)]
+
+    return LatticeDesc, CaseData, KernelData, KernelFunctions
+
+
+class Library:
+    """One loaded build of the library ("dp": PdfType double, "sp": float)."""
+
+    def __init__(self, precision="dp"):
+        path = library_path(precision)
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built; run make -C lbm-bench_amd")
+        self.precision = precision
+        self.pdf = ctypes.c_float if precision == "sp" else ctypes.c_double
+        self.lib = ctypes.CDLL(path)
+        (self.LatticeDesc, self.CaseData, self.KernelData,
+         self.KernelFunctions) = _structs(self.pdf)
+        L = self.lib
+        LDP = ctypes.POINTER(self.LatticeDesc)
+        KDP = ctypes.POINTER(self.KernelData)
+        CDP = ctypes.POINTER(self.CaseData)
+        dp = ctypes.POINTER(c_double)
+        L.geometryCreate.argtypes = [c_char_p, I3, I3, LDP]
+        L.geometrySetSeed.argtypes = [ctypes.c_uint64]
+        L.geometryDestroy.argtypes = [LDP]
+        L.caseDataDefaults.argtypes = [CDP]
+        L.lbmGetNodes.argtypes = [LDP, KDP, dp]
+        L.lbmSetNodes.argtypes = [LDP, KDP, dp]
+        L.lbmInitPdfs.argtypes = [LDP, KDP, c_int, ctypes.c_uint64, c_double, c_double]
+        L.lbmMassMomentum.argtypes = [LDP, KDP, dp]
+        L.lbmCompareMomentum.argtypes = [LDP, KDP, KDP, dp]
+        L.lbmRunSteps.argtypes = [LDP, KDP, CDP, c_int]
+        L.lbmRunSteps.restype = c_double
+        L.lbmDeviceBytes.argtypes = [KDP]
+        L.lbmDeviceBytes.restype = ctypes.c_size_t
+        L.lbmCopyBandwidth.argtypes = [ctypes.c_size_t, c_int]
+        L.lbmCopyBandwidth.restype = c_double
+        L.lbmDeviceSynchronize.restype = c_int
+        L.lbmDeviceInfo.argtypes = [c_char_p, c_int, ctypes.POINTER(c_int),
+                                    ctypes.POINTER(ctypes.c_size_t)]
+        L.kernelDensity.argtypes = [KDP, LDP]
+        L.kernelDensity.restype = self.pdf
+        L.kernelVerify.argtypes = [LDP, KDP, CDP, ctypes.POINTER(self.pdf)]
+        L.kernelSetInitialDensity.argtypes = [LDP, KDP, CDP]
+        L.kernelSetInitialVelocity.argtypes = [LDP, KDP, CDP]
+        L.kernelAddBodyForce.argtypes = [KDP, LDP, CDP]
+        self.n_kernels = c_int.in_dll(L, "G_N_KERNELS").value
+        self.registry = (self.KernelFunctions * self.n_kernels).in_dll(L, "GKernels")
+
+    def kernel_names(self):
+        return [self.registry[i].Name.decode() for i in range(self.n_kernels)]
+
+    def device_info(self):
+        name = ctypes.create_string_buffer(256)
+        cus = c_int(0)
+        mem = ctypes.c_size_t(0)
+        rc = self.lib.lbmDeviceInfo(name, 256, ctypes.byref(cus), ctypes.byref(mem))
+        if rc != 0:
+            raise RuntimeError("no HIP device")
+        return name.value.decode(), cus.value, mem.value
+
+    def copy_bandwidth(self, nbytes, reps):
+        return self.lib.lbmCopyBandwidth(nbytes, reps)
+
+    def synchronize(self):
+        if self.lib.lbmDeviceSynchronize() != 0:
+            raise RuntimeError("hipDeviceSynchronize failed")
+
+
+class Lattice:
+    """A lattice bound to one registered kernel (Init ... Deinit)."""
+
+    def __init__(self, lib, kernel, geometry, dims, periodic=(0, 0, 0), omega=1.0,
+                 xforce=0.0, body_force=False, magic=1.0 / 12.0, storage="native",
+                 scale=0.0, rho_ref=1.0, seed=0):
+        self.lib = lib
+        L = lib.lib
+        self.ld = lib.LatticeDesc()
+        L.geometrySetSeed(seed)
+        per = I3(*periodic)
+        L.geometryCreate(geometry.encode(), I3(*dims), per, ctypes.byref(self.ld))
+        self.dims = tuple(dims)
+        self.periodic = (self.ld.PeriodicX, self.ld.PeriodicY, self.ld.PeriodicZ)
+        self.cd = lib.CaseData()
+        self.cd.MaxIterations = 0
+        self.cd.RhoIn = 1.0
+        self.cd.RhoOut = 1.0
+        self.cd.StatisticsModulus = 100
+        L.caseDataDefaults(ctypes.byref(self.cd))
+        self.cd.Omega = omega
+        self.cd.XForce = xforce
+        self.cd.BodyForce = int(body_force)
+        self.cd.MagicParam = magic
+        self.cd.Storage = STORAGE[storage]
+        self.cd.StorageScale = scale
+        self.cd.RhoRef = rho_ref
+        names = lib.kernel_names()
+        if kernel not in names:
+            L.geometryDestroy(ctypes.byref(self.ld))
+            raise ValueError(f"unknown kernel {kernel}")
+        self.kf = lib.registry[names.index(kernel)]
+        self.kd = ctypes.POINTER(lib.KernelData)()
+        self.kf.Init(ctypes.byref(self.ld), ctypes.byref(self.kd), ctypes.byref(self.cd))
+        self.kernel = kernel
+
+    # -- geometry -------------------------------------------------------
+    def lattice_flags(self):
+        n = self.dims[0] * self.dims[1] * self.dims[2]
+        a = np.ctypeslib.as_array(self.ld.Lattice, shape=(n,))
+        return a.reshape(self.dims[2], self.dims[1], self.dims[0]).copy()
+
+    def fluid_mask(self):
+        """bool (nX, nY, nZ): True on non-obstacle cells"""
+        return self.lattice_flags().transpose(2, 1, 0) != 0
+
+    @property
+    def n_fluid(self):
+        return self.ld.nFluid
+
+    @property
+    def loop_balance(self):
+        return self.kd.contents.LoopBalance
+
+    @property
+    def iteration(self):
+        return self.kd.contents.Iteration
+
+    # -- state ----------------------------------------------------------
+    def set_nodes(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        assert arr.shape == self.dims + (19,)
+        self.lib.lib.lbmSetNodes(ctypes.byref(self.ld), self.kd,
+                                 arr.ctypes.data_as(ctypes.POINTER(c_double)))
+
+    def get_nodes(self):
+        out = np.empty(self.dims + (19,), dtype=np.float64)
+        self.lib.lib.lbmGetNodes(ctypes.byref(self.ld), self.kd,
+                                 out.ctypes.data_as(ctypes.POINTER(c_double)))
+        return out
+
+    def get_node(self, x, y, z):
+        buf = (self.lib.pdf * 19)()
+        self.kd.contents.GetNode(self.kd, x, y, z, buf)
+        return np.array(buf[:], dtype=np.float64)
+
+    def set_node(self, x, y, z, values):
+        buf = (self.lib.pdf * 19)(*[float(v) for v in values])
+        self.kd.contents.SetNode(self.kd, x, y, z, buf)
+
+    def init(self, mode="random", seed=0, rho=1.0, amp=1e-3):
+        self.lib.lib.lbmInitPdfs(ctypes.byref(self.ld), self.kd, INIT[mode], seed, rho, amp)
+
+    def init_reference(self):
+        """LBM-Bench's kernelSetInitialDensity + kernelSetInitialVelocity"""
+        L = self.lib.lib
+        L.kernelSetInitialDensity(ctypes.byref(self.ld), self.kd, ctypes.byref(self.cd))
+        L.kernelSetInitialVelocity(ctypes.byref(self.ld), self.kd, ctypes.byref(self.cd))
+
+    # -- stepping -------------------------------------------------------
+    def run(self, steps):
+        """runs `steps` steps through kd->Kernel; returns hipEvent seconds"""
+        self.cd.MaxIterations = steps
+        self.kd.contents.Kernel(ctypes.byref(self.ld), self.kd, ctypes.byref(self.cd))
+        return self.kd.contents.Duration
+
+    def add_body_force(self):
+        self.lib.lib.kernelAddBodyForce(self.kd, ctypes.byref(self.ld), ctypes.byref(self.cd))
+
+    # -- checks ---------------------------------------------------------
+    def mass_momentum(self):
+        out = np.zeros(4)
+        self.lib.lib.lbmMassMomentum(ctypes.byref(self.ld), self.kd,
+                                     out.ctypes.data_as(ctypes.POINTER(c_double)))
+        return out
+
+    def density(self):
+        return float(self.lib.lib.kernelDensity(self.kd, ctypes.byref(self.ld)))
+
+    def verify(self):
+        norm = self.lib.pdf(0.0)
+        self.lib.lib.kernelVerify(ctypes.byref(self.ld), self.kd, ctypes.byref(self.cd),
+                                  ctypes.byref(norm))
+        return float(norm.value)
+
+    def compare_momentum(self, other):
+        out = np.zeros(4)
+        self.lib.lib.lbmCompareMomentum(ctypes.byref(self.ld), self.kd, other.kd,
+                                        out.ctypes.data_as(ctypes.POINTER(c_double)))
+        return out
+
+    def device_bytes(self):
+        return self.lib.lib.lbmDeviceBytes(self.kd)
+
+    def close(self):
+        if self.kd:
+            self.kf.Deinit(ctypes.byref(self.ld), ctypes.byref(self.kd))
+            self.kd = ctypes.POINTER(self.lib.KernelData)()
+        if self.ld.Lattice:
+            self.lib.lib.geometryDestroy(ctypes.byref(self.ld))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -1,0 +1,718 @@
+/* lbm_oracle.c -- CPU restatement of LBM-Bench's D3Q19 TRT
+ * stream-and-collide path with half-way bounce-back.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle: tests/, the smoke
+ * check and bench.py's cpu_baseline leg load it as liboracle.so to check
+ * (never to replace) the GPU path.  The product never links it.
+ *
+ * It follows the reference's own mechanism, not the GPU path's: a lattice
+ * with one halo layer, explicit bounce-back/periodic copy lists and the
+ * per-variant node accessors.  The GPU path folds all of that into a
+ * masked gather/scatter, so agreement between the two is a real check.
+ *
+ * Differences from the reference, each a parameter the reference fixes:
+ *   - Lambda (magic) is an argument; the reference hard-codes 1/12
+ *     (src/kernel_pull_soa.c:20).  magic = 1/12 reproduces it.
+ *   - The body force is an argument; the reference compiles it in with
+ *     -DVERIFICATION (src/kernel_pull_soa.c:206-209).
+ *   - Indices are 64-bit (the reference's int overflows beyond 480^3,
+ *     src/lbm.h:113-121).
+ *
+ * Parity status: the collision, streaming, bounce-back, force, AA parity,
+ * density and verify functions are pinned against the compiled reference
+ * (oracle/_ref, tests/test_oracle_vs_reference.py) and against golden
+ * fixtures generated from it (tests/golden/).  The Lambda dependence
+ * (magic != 1/12) is pinned by closed forms: Lambda = (1/omega - 1/2)^2
+ * reduces TRT to BGK, and Lambda = 3/16 makes steady Poiseuille flow exact
+ * (tests/test_oracle_closed_forms.py).  The "spheres-N" geometry and the
+ * seeded initialiser have no reference counterpart; they are specified
+ * here and pinned by brute-force restatements in the tests.
+ */
+#include "lbm_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <strings.h>
+
+/* direction ids and velocity set (src/lbm.h:20-38, src/lbm.c:20-29) */
+enum { N = 0, S, E, W, NE, SE, NW, SW, T, TN, TE, TW, TS, B, BS, BN, BW, BE, C, Q };
+static const int CX[Q]  = { 0, 0, 1, -1, 1, 1, -1, -1, 0, 0, 1, -1, 0, 0, 0, 0, -1, 1, 0 };
+static const int CY[Q]  = { 1, -1, 0, 0, 1, -1, 1, -1, 0, 1, 0, 0, -1, 0, -1, 1, 0, 0, 0 };
+static const int CZ[Q]  = { 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, -1, -1, -1, -1, -1, 0 };
+static const int INV[Q] = { S, N, W, E, SW, NW, SE, NE, B, BS, BW, BE, BN, T, TN, TS, TE, TW, C };
+
+enum { OBST = 0, FLUID = 1 };
+enum { P_PUSH = 0, P_PULL = 1, P_AA = 2 };
+enum { L_SOA = 0, L_AOS = 1 };
+
+/* --------------------------------------------------------------------- */
+/* seeded inputs: splitmix64 (Steele, Lea & Flood 2014) finalizer chained */
+/* over (seed, a, b); u01 takes the top 53 bits.                          */
+/* --------------------------------------------------------------------- */
+static uint64_t mix64(uint64_t x)
+{
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+uint64_t oracle_hash(uint64_t seed, uint64_t a, uint64_t b)
+{
+  return mix64(mix64(mix64(seed) ^ a) ^ b);
+}
+
+static double u01(uint64_t h) { return (double)(h >> 11) * (1.0 / 9007199254740992.0); }
+
+/* --------------------------------------------------------------------- */
+/* geometry: src/lbm.c:52-187                                             */
+/* --------------------------------------------------------------------- */
+static inline int64_t lat_index(const int n[3], int x, int y, int z)
+{
+  return ((int64_t)z * n[1] + y) * n[0] + x;
+}
+
+#define SPHERE_SALT 0x5350484552455321ull
+
+int oracle_geometry(const char *type, const int dims[3], int per[3], uint64_t seed,
+    int *lat, int counts[2])
+{
+  enum { G_BOX, G_CHANNEL, G_PIPE, G_BLOCKS, G_FLUID, G_SPHERES } g;
+  int blockSize = 8, nSpheres = 0;
+  if (strncasecmp("channel", type, 7) == 0)
+    g = G_CHANNEL;
+  else if (strncasecmp("box", type, 3) == 0)
+    g = G_BOX;
+  else if (strncasecmp("pipe", type, 4) == 0)
+    g = G_PIPE;
+  else if (strncasecmp("blocks", type, 6) == 0) {
+    g = G_BLOCKS;
+    if (strlen(type) > 7) {
+      blockSize = atoi(type + 7);
+      if (blockSize <= 0)
+        blockSize = 8;
+    }
+  } else if (strncasecmp("fluid", type, 5) == 0)
+    g = G_FLUID;
+  else if (strncasecmp("spheres", type, 7) == 0) {
+    g        = G_SPHERES;
+    nSpheres = strlen(type) > 8 ? atoi(type + 8) : 0;
+  } else
+    return -1;
+
+  const int nx = dims[0], ny = dims[1], nz = dims[2];
+  const int64_t ncells = (int64_t)nx * ny * nz;
+  for (int64_t i = 0; i < ncells; ++i)
+    lat[i] = FLUID;
+
+  /* forced periodicity (src/lbm.c:96-102) */
+  if (g == G_CHANNEL || g == G_BLOCKS || g == G_PIPE || g == G_SPHERES)
+    per[0] = 1;
+  else if (g == G_FLUID)
+    per[0] = per[1] = per[2] = 1;
+
+  /* boundary layers of non-periodic axes are obstacles (src/lbm.c:104-129) */
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        int onFace = (!per[0] && (x == 0 || x == nx - 1)) ||
+                     (!per[1] && (y == 0 || y == ny - 1)) ||
+                     (!per[2] && (z == 0 || z == nz - 1));
+        if (onFace)
+          lat[lat_index(dims, x, y, z)] = OBST;
+      }
+
+  if (g == G_PIPE) { /* src/lbm.c:131-142 */
+    double cz = nz / 2.0 - 0.5, cy = ny / 2.0 - 0.5;
+    double dmin = ny < nz ? ny : nz;
+    double r2   = (dmin / 2 - 1) * (dmin / 2 - 1);
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        if ((z - cz) * (z - cz) + (y - cy) * (y - cy) >= r2)
+          for (int x = 0; x < nx; ++x)
+            lat[lat_index(dims, x, y, z)] = OBST;
+  } else if (g == G_BLOCKS) { /* src/lbm.c:143-156 */
+    int nb = blockSize;
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x)
+          if ((z % (2 * nb)) >= nb && (y % (2 * nb)) >= nb && (x % (2 * nb)) >= nb)
+            lat[lat_index(dims, x, y, z)] = OBST;
+  } else if (g == G_SPHERES) {
+    /* sphere k: centre U[0,n) per axis, radius U[4,12]; a cell is solid when
+     * its integer position lies within r of the centre, the bounding box
+     * wrapping around periodic axes and clipped on the others. */
+    const int ncoord[3] = { nx, ny, nz };
+    for (int k = 0; k < nSpheres; ++k) {
+      double c[3], r;
+      for (int a = 0; a < 3; ++a)
+        c[a] = u01(oracle_hash(seed ^ SPHERE_SALT, (uint64_t)k, (uint64_t)a)) * ncoord[a];
+      r = 4.0 + 8.0 * u01(oracle_hash(seed ^ SPHERE_SALT, (uint64_t)k, 3));
+      int lo[3], hi[3];
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = (int)floor(c[a] - r);
+        hi[a] = (int)ceil(c[a] + r);
+      }
+      for (int z = lo[2]; z <= hi[2]; ++z)
+        for (int y = lo[1]; y <= hi[1]; ++y)
+          for (int x = lo[0]; x <= hi[0]; ++x) {
+            double dx = x - c[0], dy = y - c[1], dz = z - c[2];
+            if (dx * dx + dy * dy + dz * dz > r * r)
+              continue;
+            int p[3] = { x, y, z }, ok = 1;
+            for (int a = 0; a < 3; ++a) {
+              if (p[a] < 0 || p[a] >= ncoord[a]) {
+                if (!per[a]) {
+                  ok = 0;
+                  break;
+                }
+                p[a] = ((p[a] % ncoord[a]) + ncoord[a]) % ncoord[a];
+              }
+            }
+            if (ok)
+              lat[lat_index(dims, p[0], p[1], p[2])] = OBST;
+          }
+    }
+  }
+
+  int64_t nf = 0, no = 0;
+  for (int64_t i = 0; i < ncells; ++i) {
+    if (lat[i] == OBST)
+      ++no;
+    else
+      ++nf;
+  }
+  counts[0] = (int)nf;
+  counts[1] = (int)no;
+  return 0;
+}
+
+/* --------------------------------------------------------------------- */
+/* TRT collision, src/kernel_pull_soa.c:18-28 (constants) and :118-198    */
+/* (update).  f holds the 19 pre-collision values, o receives the 19      */
+/* post-collision values, both indexed by direction.                      */
+/* --------------------------------------------------------------------- */
+typedef struct {
+  double we, wo, w0, w1, w2, w1x3, w2x3, w1nh, w2nh;
+} trt_t;
+
+static void trt_setup(trt_t *p, double omega, double magic)
+{
+  p->we   = omega;
+  p->wo   = 1.0 / (0.5 + magic / (1.0 / omega - 0.5));
+  p->w0   = 1.0 / 3.0;
+  p->w1   = 1.0 / 18.0;
+  p->w2   = 1.0 / 36.0;
+  p->w1x3 = p->w1 * 3.0;
+  p->w1nh = p->w1 * 9.0 / 2.0;
+  p->w2x3 = p->w2 * 3.0;
+  p->w2nh = p->w2 * 9.0 / 2.0;
+}
+
+/* summation orders of the moments as the reference writes them */
+static const int RHO_ORDER[Q] = { C, N, E, S, W, NE, SE, SW, NW, T, TN, TE, TS, TW, B, BN, BE, BS, BW };
+static const int JX_PLUS[5] = { E, NE, SE, TE, BE }, JX_MINUS[5] = { W, NW, SW, TW, BW };
+static const int JY_PLUS[5] = { N, NE, NW, TN, BN }, JY_MINUS[5] = { S, SE, SW, TS, BS };
+static const int JZ_PLUS[5] = { T, TE, TW, TN, TS }, JZ_MINUS[5] = { B, BE, BW, BN, BS };
+
+/* the nine (i, inv i) pairs in the reference's order; u_i = c_i . j */
+static const int PAIR[9][2] = { { N, S }, { E, W }, { T, B }, { NW, SE }, { NE, SW },
+  { TW, BE }, { TE, BW }, { TS, BN }, { TN, BS } };
+
+static double sum5(const double *f, const int *plus, const int *minus)
+{
+  double s = f[plus[0]];
+  for (int k = 1; k < 5; ++k)
+    s += f[plus[k]];
+  for (int k = 0; k < 5; ++k)
+    s -= f[minus[k]];
+  return s;
+}
+
+static void trt_collide(const double *f, double *o, const trt_t *p)
+{
+  double ux = sum5(f, JX_PLUS, JX_MINUS);
+  double uy = sum5(f, JY_PLUS, JY_MINUS);
+  double uz = sum5(f, JZ_PLUS, JZ_MINUS);
+  double rho = f[RHO_ORDER[0]];
+  for (int k = 1; k < Q; ++k)
+    rho += f[RHO_ORDER[k]];
+  double D = rho - (ux * ux + uy * uy + uz * uz) * 3.0 / 2.0;
+
+  o[C] = f[C] - p->we * (f[C] - p->w0 * D);
+  for (int k = 0; k < 9; ++k) {
+    int i = PAIR[k][0], j = PAIR[k][1];
+    double ui = CX[i] * ux + CY[i] * uy + CZ[i] * uz; /* exact: coefficients are 0, +-1 */
+    /* the reference forms -ux+uy as (-ux)+uy; the sum of two terms is the
+     * same either way, and one- or two-term sums are exact re-statements */
+    int first = k < 3;
+    double wX3 = first ? p->w1x3 : p->w2x3, wNH = first ? p->w1nh : p->w2nh;
+    double wIndep = (first ? p->w1 : p->w2) * D;
+    double even   = p->we * (0.5 * (f[i] + f[j]) - ui * ui * wNH - wIndep);
+    double odd    = p->wo * (0.5 * (f[i] - f[j]) - ui * wX3);
+    o[i]          = f[i] - even - odd;
+    o[j]          = f[j] - even + odd;
+  }
+}
+
+/* --------------------------------------------------------------------- */
+/* lattice state with a halo, src/lbm.c:400-599                          */
+/* --------------------------------------------------------------------- */
+typedef struct {
+  int prop, layout, blocked;
+  int n[3];
+  int64_t g[3], gcells;
+  double *pdf[2];
+  int64_t *bbSrc, *bbDst, nbb;
+  const int *lat;
+  int per[3];
+  int iteration;
+} ost_t;
+
+static inline int64_t pidx(const ost_t *s, int64_t x, int64_t y, int64_t z, int d)
+{
+  if (s->layout == L_SOA)
+    return d * s->gcells + (x * s->g[1] + y) * s->g[2] + z;
+  return ((x * s->g[1] + y) * s->g[2] + z) * Q + d;
+}
+
+static inline int obstacle(const ost_t *s, int x, int y, int z)
+{
+  return s->lat[lat_index(s->n, x, y, z)] == OBST;
+}
+
+/* Bounce-back / periodic copy lists, the rule of src/lbm.c:465-568:
+ * for every fluid cell and direction, n = x - c (pull) or x + c (push, AA).
+ * Leaving through a periodic face: wrap to p; p obstacle -> bounce-back
+ * entry, else a periodic copy.  In range and obstacle -> bounce-back entry.
+ * Leaving through a non-periodic face -> nothing. */
+static void build_lists(ost_t *s)
+{
+  const int lx = s->n[0], ly = s->n[1], lz = s->n[2];
+  for (int pass = 0; pass < 2; ++pass) {
+    int64_t cnt = 0;
+    for (int x = 0; x < lx; ++x)
+      for (int y = 0; y < ly; ++y)
+        for (int z = 0; z < lz; ++z) {
+          if (obstacle(s, x, y, z))
+            continue;
+          for (int d = 0; d < Q; ++d) {
+            int sg = s->prop == P_PULL ? -1 : 1;
+            int nx = x + sg * CX[d], ny = y + sg * CY[d], nz = z + sg * CZ[d];
+            int outX = nx < 0 || nx >= lx, outY = ny < 0 || ny >= ly, outZ = nz < 0 || nz >= lz;
+            int64_t src, dst;
+            if ((outX && s->per[0]) || (outY && s->per[1]) || (outZ && s->per[2])) {
+              int px = nx < 0 ? lx - 1 : (nx >= lx ? 0 : nx);
+              int py = ny < 0 ? ly - 1 : (ny >= ly ? 0 : ny);
+              int pz = nz < 0 ? lz - 1 : (nz >= lz ? 0 : nz);
+              if (obstacle(s, px, py, pz)) {
+                if (s->prop == P_PULL) {
+                  src = pidx(s, x + 1, y + 1, z + 1, INV[d]);
+                  dst = pidx(s, nx + 1, ny + 1, nz + 1, d);
+                } else {
+                  src = pidx(s, nx + 1, ny + 1, nz + 1, d);
+                  dst = pidx(s, x + 1, y + 1, z + 1, INV[d]);
+                }
+              } else {
+                if (s->prop == P_PULL) {
+                  src = pidx(s, px + 1, py + 1, pz + 1, d);
+                  dst = pidx(s, nx + 1, ny + 1, nz + 1, d);
+                } else {
+                  src = pidx(s, nx + 1, ny + 1, nz + 1, d);
+                  dst = pidx(s, px + 1, py + 1, pz + 1, d);
+                }
+              }
+            } else if (outX || outY || outZ) {
+              continue;
+            } else if (obstacle(s, nx, ny, nz)) {
+              if (s->prop == P_PULL) {
+                src = pidx(s, x + 1, y + 1, z + 1, INV[d]);
+                dst = pidx(s, nx + 1, ny + 1, nz + 1, d);
+              } else {
+                src = pidx(s, nx + 1, ny + 1, nz + 1, d);
+                dst = pidx(s, x + 1, y + 1, z + 1, INV[d]);
+              }
+            } else {
+              continue;
+            }
+            if (pass == 1) {
+              s->bbSrc[cnt] = src;
+              s->bbDst[cnt] = dst;
+            }
+            ++cnt;
+          }
+        }
+    if (pass == 0) {
+      s->nbb   = cnt;
+      s->bbSrc = (int64_t *)malloc(sizeof(int64_t) * (size_t)(cnt + 1));
+      s->bbDst = (int64_t *)malloc(sizeof(int64_t) * (size_t)(cnt + 1));
+    }
+  }
+}
+
+/* node accessors: which stored slot holds logical f_d of cell (x,y,z)
+ * (src/lbm.c:193-294).  AA: even Iteration -> slot inv d of x - c_d,
+ * otherwise (odd, or -1 before the first step) the local slot d. */
+static int64_t node_slot(const ost_t *s, int x, int y, int z, int d)
+{
+  int hx = x + 1, hy = y + 1, hz = z + 1;
+  switch (s->prop) {
+  case P_PUSH:
+    return pidx(s, hx, hy, hz, d);
+  case P_PULL:
+    return pidx(s, hx - CX[d], hy - CY[d], hz - CZ[d], d);
+  default:
+    if (s->iteration % 2 == 0)
+      return pidx(s, hx - CX[d], hy - CY[d], hz - CZ[d], INV[d]);
+    return pidx(s, hx, hy, hz, d);
+  }
+}
+
+/* body force after streaming, src/lbm.c:902-922 */
+static void add_force(ost_t *s, double *a, double xforce)
+{
+  const double w[Q] = { 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 36, 1.0 / 36, 1.0 / 36,
+    1.0 / 36, 1.0 / 18, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 18, 1.0 / 36, 1.0 / 36,
+    1.0 / 36, 1.0 / 36, 1.0 / 3 };
+  double fd[Q];
+  for (int d = 0; d < Q; ++d)
+    fd[d] = 3.0 * w[d] * CX[d] * xforce;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int x = 0; x < s->n[0]; ++x)
+    for (int y = 0; y < s->n[1]; ++y)
+      for (int z = 0; z < s->n[2]; ++z) {
+        if (obstacle(s, x, y, z))
+          continue;
+        for (int d = 0; d < Q; ++d)
+          a[node_slot(s, x, y, z, d)] += fd[d];
+      }
+}
+
+static void copy_lists(const ost_t *s, double *a, int forward)
+{
+  if (forward)
+    for (int64_t i = 0; i < s->nbb; ++i)
+      a[s->bbDst[i]] = a[s->bbSrc[i]];
+  else
+    for (int64_t i = 0; i < s->nbb; ++i)
+      a[s->bbSrc[i]] = a[s->bbDst[i]];
+}
+
+/* one two-lattice step: push (local read, write to x + c) or pull (read
+ * from x - c, local write) over every interior cell, obstacles included
+ * (src/kernel_push_soa.c:94-199, src/kernel_pull_soa.c:94-199); the
+ * blocked variants visit the same cells in 8^3 tiles
+ * (src/kernel_blk_pull_soa.c:38-47). */
+static void cell_twolattice(const ost_t *s, const double *src, double *dst, int x, int y,
+    int z, const trt_t *p)
+{
+  double f[Q], o[Q];
+  if (s->prop == P_PULL) {
+    for (int d = 0; d < Q; ++d)
+      f[d] = src[pidx(s, x - CX[d], y - CY[d], z - CZ[d], d)];
+    trt_collide(f, o, p);
+    for (int d = 0; d < Q; ++d)
+      dst[pidx(s, x, y, z, d)] = o[d];
+  } else {
+    for (int d = 0; d < Q; ++d)
+      f[d] = src[pidx(s, x, y, z, d)];
+    trt_collide(f, o, p);
+    for (int d = 0; d < Q; ++d)
+      dst[pidx(s, x + CX[d], y + CY[d], z + CZ[d], d)] = o[d];
+  }
+}
+
+static void step_twolattice(ost_t *s, const double *src, double *dst, const trt_t *p)
+{
+  const int nx = s->n[0], ny = s->n[1], nz = s->n[2];
+  if (!s->blocked) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int x = 1; x <= nx; ++x)
+      for (int y = 1; y <= ny; ++y)
+        for (int z = 1; z <= nz; ++z)
+          cell_twolattice(s, src, dst, x, y, z, p);
+  } else {
+    const int bk = 8;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int bx = 1; bx <= nx; bx += bk)
+      for (int by = 1; by <= ny; by += bk)
+        for (int bz = 1; bz <= nz; bz += bk) {
+          int ex = bx + bk <= nx + 1 ? bx + bk : nx + 1;
+          int ey = by + bk <= ny + 1 ? by + bk : ny + 1;
+          int ez = bz + bk <= nz + 1 ? bz + bk : nz + 1;
+          for (int x = bx; x < ex; ++x)
+            for (int y = by; y < ey; ++y)
+              for (int z = bz; z < ez; ++z)
+                cell_twolattice(s, src, dst, x, y, z, p);
+        }
+  }
+}
+
+/* AA pattern on one array (src/kernel_aa_soa.c:36-286).
+ * Even: fluid cells only, read slot d locally, write post-collision f_d
+ * to the local slot inv d.  Odd: every interior cell, read f_d from slot
+ * inv d of x - c_d, write post-collision f_d to slot d of x + c_d.
+ * Each cell owns the slots it reads and writes, so cells are independent. */
+static void step_aa(ost_t *s, double *a, int odd, const trt_t *p)
+{
+  const int nx = s->n[0], ny = s->n[1], nz = s->n[2];
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int x = 1; x <= nx; ++x)
+    for (int y = 1; y <= ny; ++y)
+      for (int z = 1; z <= nz; ++z) {
+        double f[Q], o[Q];
+        if (!odd) {
+          if (obstacle(s, x - 1, y - 1, z - 1))
+            continue;
+          for (int d = 0; d < Q; ++d)
+            f[d] = a[pidx(s, x, y, z, d)];
+          trt_collide(f, o, p);
+          for (int d = 0; d < Q; ++d)
+            a[pidx(s, x, y, z, INV[d])] = o[d];
+        } else {
+          for (int d = 0; d < Q; ++d)
+            f[d] = a[pidx(s, x - CX[d], y - CY[d], z - CZ[d], INV[d])];
+          trt_collide(f, o, p);
+          for (int d = 0; d < Q; ++d)
+            a[pidx(s, x + CX[d], y + CY[d], z + CZ[d], d)] = o[d];
+        }
+      }
+}
+
+static int parse_kernel(const char *name, int *prop, int *layout, int *blocked)
+{
+  static const struct {
+    const char *name;
+    int prop, layout, blocked;
+  } tab[] = {
+    { "pull-soa", P_PULL, L_SOA, 0 },
+    { "pull-aos", P_PULL, L_AOS, 0 },
+    { "push-soa", P_PUSH, L_SOA, 0 },
+    { "push-aos", P_PUSH, L_AOS, 0 },
+    { "blk-push-soa", P_PUSH, L_SOA, 1 },
+    { "blk-pull-soa", P_PULL, L_SOA, 1 },
+    { "aa-soa", P_AA, L_SOA, 0 },
+  };
+  for (size_t i = 0; i < sizeof(tab) / sizeof(tab[0]); ++i)
+    if (!strcasecmp(name, tab[i].name)) {
+      *prop    = tab[i].prop;
+      *layout  = tab[i].layout;
+      *blocked = tab[i].blocked;
+      return 0;
+    }
+  return -1;
+}
+
+/* One run.
+ *   kernel      one of the seven reference names (src/lbm.c:33-41)
+ *   lattice     flags from oracle_geometry
+ *   periodic    periodicity after geometry forcing
+ *   init        logical PDFs set on every non-obstacle cell before the run
+ *               (as SetNode with Iteration -1), or NULL for all zero
+ *   iterations  requested steps; aa-soa runs them in pairs, so an odd
+ *               count runs one more (src/kernel_aa_soa.c:36)
+ *   bodyforce   1: add 3 w_i c_ix xforce after each step's bounce-back
+ *   out         logical PDFs after the run (obstacle cells: 0)
+ *   steps_done  steps actually run
+ * Returns 0, or -1 for an unknown kernel. */
+int oracle_run(const char *kernel, const int dims[3], const int *lattice,
+    const int periodic[3], const double *init, int iterations, double omega, double magic,
+    double xforce, int bodyforce, double *out, int *steps_done)
+{
+  ost_t s;
+  memset(&s, 0, sizeof(s));
+  if (parse_kernel(kernel, &s.prop, &s.layout, &s.blocked))
+    return -1;
+  for (int a = 0; a < 3; ++a) {
+    s.n[a]   = dims[a];
+    s.g[a]   = dims[a] + 2;
+    s.per[a] = periodic[a];
+  }
+  s.gcells    = s.g[0] * s.g[1] * s.g[2];
+  s.lat       = lattice;
+  s.iteration = -1;
+  int narrays = s.prop == P_AA ? 1 : 2;
+  for (int i = 0; i < narrays; ++i)
+    s.pdf[i] = (double *)calloc((size_t)(s.gcells * Q), sizeof(double));
+  build_lists(&s);
+
+  const int nx = dims[0], ny = dims[1], nz = dims[2];
+  if (init) {
+    for (int x = 0; x < nx; ++x)
+      for (int y = 0; y < ny; ++y)
+        for (int z = 0; z < nz; ++z) {
+          if (obstacle(&s, x, y, z))
+            continue;
+          const double *v = init + (((int64_t)x * ny + y) * nz + z) * Q;
+          for (int d = 0; d < Q; ++d)
+            s.pdf[0][node_slot(&s, x, y, z, d)] = v[d];
+        }
+  }
+
+  trt_t p;
+  trt_setup(&p, omega, magic);
+  double *final = s.pdf[0];
+  int done      = 0;
+  if (s.prop == P_AA) {
+    for (int iter = 0; iter < iterations; iter += 2) {
+      step_aa(&s, s.pdf[0], 0, &p);
+      copy_lists(&s, s.pdf[0], 0);
+      s.iteration = iter;
+      if (bodyforce)
+        add_force(&s, s.pdf[0], xforce);
+      step_aa(&s, s.pdf[0], 1, &p);
+      copy_lists(&s, s.pdf[0], 1);
+      s.iteration = iter + 1;
+      if (bodyforce)
+        add_force(&s, s.pdf[0], xforce);
+      done += 2;
+    }
+  } else {
+    double *src = s.pdf[0], *dst = s.pdf[1];
+    for (int iter = 0; iter < iterations; ++iter) {
+      step_twolattice(&s, src, dst, &p);
+      copy_lists(&s, dst, 1);
+      if (bodyforce)
+        add_force(&s, dst, xforce);
+      double *t = src;
+      src       = dst;
+      dst       = t;
+      ++done;
+    }
+    final = src;
+  }
+
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y)
+      for (int z = 0; z < nz; ++z) {
+        double *v = out + (((int64_t)x * ny + y) * nz + z) * Q;
+        for (int d = 0; d < Q; ++d)
+          v[d] = obstacle(&s, x, y, z) ? 0.0 : final[node_slot(&s, x, y, z, d)];
+      }
+  if (steps_done)
+    *steps_done = done;
+  free(s.pdf[0]);
+  free(s.pdf[1]);
+  free(s.bbSrc);
+  free(s.bbDst);
+  return 0;
+}
+
+/* --------------------------------------------------------------------- */
+/* seeded initial state                                                   */
+/* --------------------------------------------------------------------- */
+static const double WEIGHT[Q] = { 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 36, 1.0 / 36,
+  1.0 / 36, 1.0 / 36, 1.0 / 18, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 18, 1.0 / 36,
+  1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 3 };
+
+/* f_d = w_d * (rho + eps), eps = (2u - 1) * amp with u from
+ * oracle_hash(seed, cell, d), cell = (x*nY + y)*nZ + z.  The operations are
+ * written so that no two of them may be contracted into one FMA. */
+void oracle_init_random(const int dims[3], const int *lattice, uint64_t seed, double amp,
+    double rho, double *out)
+{
+  const int nx = dims[0], ny = dims[1], nz = dims[2];
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y)
+      for (int z = 0; z < nz; ++z) {
+        int64_t cell = ((int64_t)x * ny + y) * nz + z;
+        double *v    = out + cell * Q;
+        int fluid    = lattice[lat_index(dims, x, y, z)] != OBST;
+        for (int d = 0; d < Q; ++d) {
+          if (!fluid) {
+            v[d] = 0.0;
+            continue;
+          }
+          double u   = u01(oracle_hash(seed, (uint64_t)cell, (uint64_t)d));
+          volatile double t = 2.0 * u - 1.0;
+          volatile double e = t * amp;
+          volatile double r = rho + e;
+          v[d]              = WEIGHT[d] * r;
+        }
+      }
+}
+
+/* --------------------------------------------------------------------- */
+/* statistics and verification, src/lbm.c:928-1023                       */
+/* --------------------------------------------------------------------- */
+double oracle_density(const int dims[3], const int *lat, const double *pdfs)
+{
+  const int nx = dims[0], ny = dims[1], nz = dims[2];
+  double dens = 0.0;
+  int64_t nf  = 0;
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        if (lat[lat_index(dims, x, y, z)] == OBST)
+          continue;
+        const double *v = pdfs + (((int64_t)x * ny + y) * nz + z) * Q;
+        double loc      = 0.0;
+        for (int d = 0; d < Q; ++d)
+          loc += v[d];
+        dens += loc;
+        ++nf;
+      }
+  return dens / (double)nf;
+}
+
+void oracle_moments(const int dims[3], const int *lat, const double *pdfs, double out[4])
+{
+  const int nx = dims[0], ny = dims[1], nz = dims[2];
+  double m[4] = { 0, 0, 0, 0 };
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y)
+      for (int z = 0; z < nz; ++z) {
+        if (lat[lat_index(dims, x, y, z)] == OBST)
+          continue;
+        const double *v = pdfs + (((int64_t)x * ny + y) * nz + z) * Q;
+        for (int d = 0; d < Q; ++d) {
+          m[0] += v[d];
+          m[1] += CX[d] * v[d];
+          m[2] += CY[d] * v[d];
+          m[3] += CZ[d] * v[d];
+        }
+      }
+  for (int k = 0; k < 4; ++k)
+    out[k] = m[k];
+}
+
+/* Poiseuille check of src/lbm.c:964-1023: u_x at x = nX/2 (plus g/2 when
+ * the force correction is on, as under VERIFICATION), analytic parabola
+ * over z with centre nY/2 and radius nY/2 - 1, averaged over the one or two
+ * central y rows; returns the L2 norm of the relative error. */
+double oracle_verify(const int dims[3], const int *lat, const double *pdfs, double omega,
+    double xforce, int force_correction)
+{
+  const int nx = dims[0], ny = dims[1], nz = dims[2];
+  double visc   = (1.0 / omega - 0.5) / 3.0;
+  double *prof  = (double *)calloc((size_t)ny * nz, sizeof(double));
+  int xpos      = nx / 2;
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y) {
+      if (lat[lat_index(dims, xpos, y, z)] == OBST)
+        continue;
+      const double *v = pdfs + (((int64_t)xpos * ny + y) * nz + z) * Q;
+      double ux = v[E] + v[NE] + v[SE] + v[TE] + v[BE] - v[W] - v[NW] - v[SW] - v[TW] - v[BW];
+      if (force_correction)
+        ux += 0.5 * xforce;
+      prof[y * nz + z] = ux;
+    }
+  double centre = ny / 2.0;
+  double r2max  = (ny / 2.0 - 1.0) * (ny / 2.0 - 1.0);
+  int even      = ny % 2 == 0;
+  int yc        = (ny - even - 1) / 2;
+  double dev    = 0.0;
+  for (int z = 0; z < nz; ++z) {
+    double r2 = (z - centre + 0.5) * (z - centre + 0.5);
+    double an = r2 >= r2max ? 0.0 : xforce * (r2max - r2) / (2.0 * visc);
+    double avg = even ? (prof[yc * nz + z] + prof[(yc + 1) * nz + z]) / 2.0 : prof[yc * nz + z];
+    if (an != 0.0) {
+      double rel = fabs(an - avg) / an;
+      dev += rel * rel;
+    }
+  }
+  free(prof);
+  return sqrt(dev);
+}

@@ -1,0 +1,62 @@
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "lbm-bench_amd"))
+
+WEIGHTS = np.array([1 / 18] * 4 + [1 / 36] * 4 + [1 / 18] + [1 / 36] * 4 + [1 / 18]
+                   + [1 / 36] * 4 + [1 / 3])
+CX = np.array([0, 0, 1, -1, 1, 1, -1, -1, 0, 0, 1, -1, 0, 0, 0, 0, -1, 1, 0])
+CY = np.array([1, -1, 0, 0, 1, -1, 1, -1, 0, 1, 0, 0, -1, 0, -1, 1, 0, 0, 0])
+CZ = np.array([0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, -1, -1, -1, -1, -1, 0])
+INV = np.array([1, 0, 3, 2, 7, 6, 5, 4, 13, 14, 16, 17, 15, 8, 9, 12, 10, 11, 18])
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    from oracle import Oracle
+    return Oracle()
+
+
+@pytest.fixture(scope="session")
+def golden_runs():
+    return np.load(os.path.join(GOLDEN, "runs.npz"))
+
+
+@pytest.fixture(scope="session")
+def manifest():
+    with open(os.path.join(GOLDEN, "manifest.json")) as f:
+        return json.load(f)
+
+
+@pytest.fixture(scope="session")
+def gpu_lib():
+    import lbmhip
+    return lbmhip.Library("dp")
+
+
+@pytest.fixture(scope="session")
+def gpu_lib_sp():
+    import lbmhip
+    return lbmhip.Library("sp")
+
+
+def rel_err(a, b, mask):
+    """max |a - b| / max |b| over the cells selected by mask (nX, nY, nZ)"""
+    return float(np.abs(a - b)[mask].max() / np.abs(b[mask]).max())
+
+
+def moments(pdfs):
+    rho = pdfs.sum(-1)
+    j = np.stack([(pdfs * CX).sum(-1), (pdfs * CY).sum(-1), (pdfs * CZ).sum(-1)], -1)
+    return rho, j

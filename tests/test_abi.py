@@ -1,0 +1,115 @@
+"""The C-ABI library loads and exports every symbol include/*.h declares,
+and the driver keeps the reference's command-line contract (src/cli.c,
+src/main.c:93-104) for the paths that need no GPU."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+from conftest import ROOT
+
+import lbmhip
+
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("lbm.h", "lbm_hip.h")]
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"static inline[^{]*\{[^}]*\}", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*[A-Za-z_][\w \*]*?\b([A-Za-z_]\w*)\s*\(", text, flags=re.M):
+            name = m.group(1)
+            if name in ("if", "for", "while", "return", "sizeof", "void"):
+                continue
+            names.add(name)
+    return names
+
+
+def declared_variables():
+    names = set()
+    for h in HEADERS:
+        for m in re.finditer(r"^extern [^;(\n\"]*?\b([A-Za-z_]\w*)\s*(\[[^\]]*\])?;",
+                             open(h).read(), flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+@pytest.mark.parametrize("precision", ["dp", "sp"])
+def test_library_exports_every_declared_symbol(precision):
+    lib = ctypes.CDLL(lbmhip.library_path(precision))
+    funcs = declared_functions()
+    assert {"kernelInitAaSoA", "kernelPullSoA", "lbmGetNodes", "geometryCreate"} <= funcs
+    missing = [n for n in sorted(funcs | declared_variables()) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_registry_names_and_order():
+    lib = lbmhip.Library("dp")
+    assert lib.kernel_names() == list(lbmhip.KERNELS)
+
+
+def test_struct_layout_matches_header():
+    """ctypes mirror of KernelDataType / CaseDataType has the C sizes"""
+    src = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "lbm.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu\n", sizeof(KernelDataType), sizeof(CaseDataType),
+         sizeof(LatticeDescType), offsetof(KernelDataType, Dev), offsetof(CaseDataType, RhoRef));
+  return 0;
+}
+"""
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        exe = os.path.join(d, "t")
+        open(c, "w").write(src)
+        subprocess.check_call(["gcc", "-I" + os.path.join(ROOT, "include"), c, "-o", exe])
+        vals = [int(v) for v in subprocess.check_output([exe]).split()]
+    lib = lbmhip.Library("dp")
+    assert vals == [ctypes.sizeof(lib.KernelData), ctypes.sizeof(lib.CaseData),
+                    ctypes.sizeof(lib.LatticeDesc), lib.KernelData.Dev.offset,
+                    lib.CaseData.RhoRef.offset]
+
+
+def _driver(*args):
+    return subprocess.run([lbmhip.driver_path("dp"), *args], capture_output=True, text=True,
+                          timeout=60)
+
+
+def test_driver_lists_kernels():
+    r = _driver("-l")
+    assert r.returncode == 0
+    listed = [l.strip() for l in r.stdout.split("Available kernels:")[1].splitlines() if l.strip()]
+    assert listed == list(lbmhip.KERNELS)
+
+
+def test_driver_help_and_errors():
+    assert _driver("-h").returncode == 0
+    r = _driver("-t", "1")
+    assert r.returncode != 0 and "unknown option" in r.stdout
+    r = _driver("-d", "0x4x4")
+    assert r.returncode != 0 and "X dimension must be > 0" in r.stdout
+    r = _driver("-d", "4x4")
+    assert r.returncode != 0 and "Z dimension is missing" in r.stdout
+    r = _driver("-i", "0")
+    assert r.returncode != 0 and "iterations must be > 0" in r.stdout
+    r = _driver("-g", "nosuch")
+    assert r.returncode != 0 and "unknown geometry type" in r.stdout
+
+
+def test_driver_unknown_kernel():
+    r = _driver("-d", "8x8x8", "-k", "nosuch")
+    assert r.returncode == 1 and 'kernel "nosuch" not found' in r.stdout
+
+
+def test_oracle_is_not_linked_by_the_product():
+    for precision in ("dp", "sp"):
+        out = subprocess.check_output(["ldd", lbmhip.library_path(precision)], text=True)
+        assert "oracle" not in out and "libref" not in out
+        out = subprocess.check_output(["ldd", lbmhip.driver_path(precision)], text=True)
+        assert "oracle" not in out

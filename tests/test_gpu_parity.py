@@ -1,0 +1,251 @@
+"""Parity of the GPU kernels (through the lbm.h C-ABI) with the reference's
+golden fixtures and with the oracle, plus the configurations' checks:
+  64^3 channel, pull-soa, tau 0.8, Lambda 3/16, g 1e-6, 1000 steps;
+  256^3 channel, all seven variants, f_i = w_i (1 + eps), 500 steps.
+FP64 tolerance: 1e-12 relative on every fluid PDF."""
+import json
+import os
+
+import numpy as np
+import pytest
+from conftest import CX, GOLDEN, WEIGHTS, moments, rel_err
+
+from lbmhip import KERNELS, Lattice
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _flags_fluid(L):
+    return L.fluid_mask()
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_matches_reference_fixtures(gpu_lib, golden_runs, manifest, case, kernel):
+    g, dims, per = manifest["run_cases"][case]
+    dims = tuple(dims)
+    init = golden_runs[f"r{case}_init"]
+    with Lattice(gpu_lib, kernel, g, dims, per, omega=manifest["omega"],
+                 xforce=manifest["xforce"], body_force=True, magic=manifest["magic"]) as L:
+        assert (L.lattice_flags() == golden_runs[f"r{case}_flags"]).all()
+        fluid = _flags_fluid(L)
+        for n in manifest["steps"]:
+            L.set_nodes(init)
+            L.run(n)
+            done = n + (n % 2 if kernel == "aa-soa" else 0)
+            key = f"r{case}_n{done}"
+            if key not in golden_runs:
+                continue
+            err = rel_err(L.get_nodes(), golden_runs[key], fluid)
+            assert err < TOL, (kernel, n, err)
+
+
+CASES = [
+    ("channel", (12, 13, 14), (0, 0, 1)),
+    ("fluid", (10, 11, 12), (0, 0, 0)),
+    ("blocks-4", (12, 12, 12), (0, 0, 0)),
+    ("box", (10, 11, 12), (1, 0, 1)),
+    ("pipe", (12, 12, 12), (0, 0, 0)),
+    ("spheres-6", (20, 22, 70), (0, 0, 1)),
+]
+
+
+@pytest.mark.parametrize("geo,dims,per", CASES)
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_matches_oracle(gpu_lib, oracle, geo, dims, per, kernel):
+    """Lambda 3/16, with and without the force, an odd step count; the
+    spheres case has nZ = 70 (a partial 64-wide segment) and wraps spheres
+    through the periodic faces"""
+    lat, per_o, _ = oracle.geometry(geo, dims, per, 3)
+    init = oracle.init_random(dims, lat, 17, 1e-2)
+    fluid = lat.transpose(2, 1, 0) != 0
+    for force in (False, True):
+        with Lattice(gpu_lib, kernel, geo, dims, per, omega=1.7, xforce=1e-4,
+                     body_force=force, magic=3 / 16, seed=3) as L:
+            assert (L.lattice_flags() == lat).all()
+            L.set_nodes(init)
+            L.run(7)
+            exp, done = oracle.run(kernel, dims, lat, per_o, init, 7, 1.7, 3 / 16, 1e-4, force)
+            assert rel_err(L.get_nodes(), exp, fluid) < TOL
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_split_runs_equal_one_run(gpu_lib, kernel):
+    dims = (12, 14, 16)
+    with Lattice(gpu_lib, kernel, "blocks-4", dims, omega=1.3, xforce=1e-4,
+                 body_force=True, magic=3 / 16) as A, \
+         Lattice(gpu_lib, kernel, "blocks-4", dims, omega=1.3, xforce=1e-4,
+                 body_force=True, magic=3 / 16) as B:
+        A.init("random", 5, 1.0, 1e-2)
+        B.init("random", 5, 1.0, 1e-2)
+        A.run(4)
+        A.run(6)
+        B.run(10)
+        assert np.array_equal(A.get_nodes(), B.get_nodes())
+
+
+def test_device_init_equals_oracle_init(gpu_lib, oracle):
+    dims = (16, 18, 20)
+    lat, _, _ = oracle.geometry("channel", dims, (0, 0, 1))
+    exp = oracle.init_random(dims, lat, 1234, 1e-3)
+    for kernel in KERNELS:
+        with Lattice(gpu_lib, kernel, "channel", dims, (0, 0, 1)) as L:
+            L.init("random", 1234, 1.0, 1e-3)
+            assert np.array_equal(L.get_nodes(), exp), kernel
+            L.init("rest", 0, 1.0, 0.0)
+            got = L.get_nodes()[lat.transpose(2, 1, 0) != 0]
+            assert np.array_equal(got, np.broadcast_to(WEIGHTS, got.shape))
+
+
+def test_node_accessors_round_trip(gpu_lib):
+    dims = (8, 9, 10)
+    for kernel in KERNELS:
+        with Lattice(gpu_lib, kernel, "channel", dims, (0, 0, 1)) as L:
+            L.init("random", 3, 1.0, 1e-2)
+            v = np.linspace(0.1, 1.9, 19)
+            L.set_node(4, 5, 6, v)
+            assert np.array_equal(L.get_node(4, 5, 6), v)
+            full = L.get_nodes()
+            assert np.array_equal(full[4, 5, 6], v)
+            L.set_nodes(full)
+            assert np.array_equal(L.get_nodes(), full)
+
+
+def test_reductions_match_numpy(gpu_lib, oracle):
+    dims = (20, 24, 28)
+    lat, per, _ = oracle.geometry("blocks-4", dims)
+    init = oracle.init_random(dims, lat, 9, 1e-2)
+    fluid = lat.transpose(2, 1, 0) != 0
+    for kernel in ("pull-soa", "pull-aos", "aa-soa"):
+        with Lattice(gpu_lib, kernel, "blocks-4", dims, omega=1.1, xforce=1e-4,
+                     body_force=True) as L:
+            L.set_nodes(init)
+            L.run(5)
+            state = L.get_nodes()
+            rho, j = moments(state)
+            m = L.mass_momentum()
+            exp = np.array([rho[fluid].sum(), *j[fluid].sum(0)])
+            assert abs(m[0] - exp[0]) < 1e-12 * exp[0]
+            assert np.abs(m[1:] - exp[1:]).max() < 1e-12 * exp[0]
+            assert abs(L.density() - exp[0] / fluid.sum()) < 1e-14
+
+
+def test_add_body_force(gpu_lib, oracle):
+    dims = (8, 10, 12)
+    g = 1e-3
+    for kernel in ("pull-soa", "push-soa", "aa-soa"):
+        with Lattice(gpu_lib, kernel, "channel", dims, (0, 0, 1), xforce=g) as L:
+            L.init("random", 1, 1.0, 1e-2)
+            before = L.get_nodes()
+            L.add_body_force()
+            after = L.get_nodes()
+            fluid = L.fluid_mask()
+            F = 3.0 * WEIGHTS * CX * g
+            F[[2, 4, 5, 10, 17]] = [3.0 * (1 / 18) * g] + [3.0 * (1 / 36) * g] * 4
+            F[[3, 6, 7, 11, 16]] = -F[[2, 4, 5, 10, 17]]
+            assert np.array_equal(after[fluid], before[fluid] + F)
+
+
+def test_channel_64_pull_soa(gpu_lib, oracle):
+    """64^3 channel (walls at y), pull-soa, tau 0.8, Lambda 3/16, g 1e-6,
+    rest start, 1000 steps: every PDF against the oracle, mass conservation,
+    and the start-up Poiseuille profile"""
+    dims, omega, g, steps = (64, 64, 64), 1.25, 1e-6, 1000
+    lat, per, counts = oracle.geometry("channel", dims, (0, 0, 1))
+    init = oracle.init_random(dims, lat, 0, 0.0)
+    exp, _ = oracle.run("pull-soa", dims, lat, per, init, steps, omega, 3 / 16, g, True)
+    fluid = lat.transpose(2, 1, 0) != 0
+    with Lattice(gpu_lib, "pull-soa", "channel", dims, (0, 0, 1), omega=omega, xforce=g,
+                 body_force=True, magic=3 / 16) as L:
+        L.init("rest", 0, 1.0, 0.0)
+        m0 = L.mass_momentum()
+        L.run(steps)
+        got = L.get_nodes()
+        m1 = L.mass_momentum()
+    assert rel_err(got, exp, fluid) < TOL
+    assert abs(m1[0] - m0[0]) / m0[0] < 1e-13
+    # momentum gained = g per fluid cell per step minus the wall friction,
+    # which the start-up series gives: compare the mean profile over x, z
+    nu = (1 / omega - 0.5) / 3
+    H = dims[1] - 2
+    y = np.arange(dims[1])
+    eta = y - 0.5
+    series = g / (2 * nu) * eta * (H - eta)
+    for n in range(1, 400, 2):
+        series -= (4 * g * H * H / (nu * (n * np.pi) ** 3) * np.sin(n * np.pi * eta / H)
+                   * np.exp(-nu * (n * np.pi / H) ** 2 * steps))
+    ux = (got * CX).sum(-1).mean(axis=(0, 2)) - g / 2
+    fl = slice(1, dims[1] - 1)
+    assert np.abs(ux[fl] - series[fl]).max() / series[fl].max() < 2e-2
+
+
+@pytest.fixture(scope="module")
+def channel256(gpu_lib, oracle):
+    dims = (256, 256, 256)
+    lat, per, counts = oracle.geometry("channel", dims, (0, 0, 1))
+    return dims, lat, per
+
+
+def test_channel_256_all_variants(gpu_lib, oracle, channel256):
+    """256^3, the seven variants from the same seeded start, 500 steps:
+    they agree with each other, conserve mass, and the first steps agree
+    with the oracle"""
+    dims, lat, per = channel256
+    fluid = lat.transpose(2, 1, 0) != 0
+    kw = dict(omega=1.25, xforce=1e-6, body_force=True, magic=3 / 16)
+    init = oracle.init_random(dims, lat, 0, 1e-3)
+    exp5, _ = oracle.run("pull-soa", dims, lat, per, init, 4, 1.25, 3 / 16, 1e-6, True)
+    base = None
+    for kernel in KERNELS:
+        with Lattice(gpu_lib, kernel, "channel", dims, (0, 0, 1), **kw) as L:
+            L.init("random", 0, 1.0, 1e-3)
+            m0 = L.mass_momentum()
+            L.run(4)
+            assert rel_err(L.get_nodes(), exp5, fluid) < TOL, kernel
+            L.run(496)
+            m1 = L.mass_momentum()
+            assert abs(m1[0] - m0[0]) / m0[0] < 1e-13, kernel
+            state = L.get_nodes()
+            if base is None:
+                base = state
+            else:
+                assert rel_err(state, base, fluid) < TOL, kernel
+            del state
+
+
+def test_verification_case(gpu_lib):
+    """LBM-Bench's -V case through the library: kernelSetInitial*, 1000
+    steps with the force, kernelVerify; the norm the reference prints"""
+    with open(os.path.join(GOLDEN, "verification.json")) as f:
+        ver = json.load(f)
+    for kernel in KERNELS:
+        with Lattice(gpu_lib, kernel, "box", (16, 16, 16), (1, 1, 0), omega=1.0, xforce=1e-5,
+                     body_force=True) as L:
+            L.init_reference()
+            L.run(1000)
+            norm = L.verify()
+            assert abs(norm - ver["dp"][kernel]) < 1e-9 * ver["dp"][kernel], kernel
+
+
+def test_single_precision_build(gpu_lib_sp):
+    """PdfType float build against the reference's PRECISION=sp fixture"""
+    sp = np.load(os.path.join(GOLDEN, "runs_sp.npz"))
+    with open(os.path.join(GOLDEN, "manifest.json")) as f:
+        man = json.load(f)
+    with open(os.path.join(GOLDEN, "verification.json")) as f:
+        ver = json.load(f)
+    for kernel in KERNELS:
+        with Lattice(gpu_lib_sp, kernel, "channel", (6, 8, 10), (0, 0, 1), omega=man["omega"],
+                     xforce=man["xforce"], body_force=True) as L:
+            fluid = L.fluid_mask()
+            L.set_nodes(sp["init"].astype(np.float32).astype(np.float64))
+            L.run(10)
+            exp = sp[f"n10_{kernel}"]
+            assert rel_err(L.get_nodes(), exp, fluid) < 2e-5, kernel
+        with Lattice(gpu_lib_sp, kernel, "box", (16, 16, 16), (1, 1, 0), omega=1.0, xforce=1e-5,
+                     body_force=True) as L:
+            L.init_reference()
+            L.run(1000)
+            assert abs(L.verify() - ver["sp"][kernel]) < 5e-3 * ver["sp"][kernel], kernel
