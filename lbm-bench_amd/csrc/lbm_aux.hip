@@ -385,19 +385,34 @@ void launchReduceCompare(const double *partials, int n, double *out, hipStream_t
 }
 
 // ---- copy ------------------------------------------------------------------
-__global__ __launch_bounds__(256) void kCopy(const double2 *__restrict__ src,
-    double2 *__restrict__ dst, uint64_t n2)
+// 4 x 16 B per thread, non-temporal: the streaming pattern of the step
+// kernels without the arithmetic
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+__global__ __launch_bounds__(256) void kCopy(const v2d *__restrict__ src, v2d *__restrict__ dst,
+    uint64_t n2)
 {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride)
-    dst[i] = src[i];
+  const uint64_t base = (uint64_t)blockIdx.x * 1024u + threadIdx.x;
+  v2d v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint64_t i = base + 256u * k;
+    if (i < n2)
+      v[k] = __builtin_nontemporal_load(&src[i]);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint64_t i = base + 256u * k;
+    if (i < n2)
+      __builtin_nontemporal_store(v[k], &dst[i]);
+  }
 }
 
 void launchCopy(const double *src, double *dst, uint64_t n, hipStream_t s)
 {
-  const uint64_t n2 = n / 2;
-  hipLaunchKernelGGL(kCopy, dim3(256 * 64), dim3(256), 0, s, (const double2 *)src,
-      (double2 *)dst, n2);
+  const uint64_t n2     = n / 2;
+  const unsigned blocks = (unsigned)((n2 + 1023) / 1024);
+  hipLaunchKernelGGL(kCopy, dim3(blocks), dim3(256), 0, s, (const v2d *)src, (v2d *)dst, n2);
 }
 
 } // namespace lbm

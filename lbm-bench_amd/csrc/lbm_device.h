@@ -173,6 +173,47 @@ __device__ __forceinline__ bool solidAt(const MaskWin *__restrict__ mask, int se
   return (w.lo >> (z & 63)) & 1ull;
 }
 
+// stores and loads, optionally non-temporal (streaming: the data is not
+// reused by this kernel, so it should not displace lines that are)
+template <bool NT, typename S>
+__device__ __forceinline__ void putS(S *p, S v)
+{
+  if constexpr (NT) {
+    if constexpr (sizeof(S) == 8)
+      __builtin_nontemporal_store(__builtin_bit_cast(unsigned long long, v), (unsigned long long *)p);
+    else if constexpr (sizeof(S) == 4)
+      __builtin_nontemporal_store(__builtin_bit_cast(unsigned int, v), (unsigned int *)p);
+    else
+      __builtin_nontemporal_store(__builtin_bit_cast(unsigned short, v), (unsigned short *)p);
+  } else {
+    *p = v;
+  }
+}
+template <bool NT, typename S>
+__device__ __forceinline__ S getS(const S *p)
+{
+  if constexpr (NT) {
+    if constexpr (sizeof(S) == 8)
+      return __builtin_bit_cast(S, __builtin_nontemporal_load((const unsigned long long *)p));
+    else if constexpr (sizeof(S) == 4)
+      return __builtin_bit_cast(S, __builtin_nontemporal_load((const unsigned int *)p));
+    else
+      return __builtin_bit_cast(S, __builtin_nontemporal_load((const unsigned short *)p));
+  } else {
+    return *p;
+  }
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  This maps the hardware order onto a logical order in which
+// consecutive logical workgroups sit on the same XCD: a bijection of
+// [0, n) for any n.
+__device__ __forceinline__ unsigned xcdLogicalBlock(unsigned b, unsigned n)
+{
+  const unsigned xcd = b % 8u, pos = b / 8u, q = n / 8u, r = n % 8u;
+  return xcd < r ? xcd * (q + 1u) + pos : r * (q + 1u) + (xcd - r) * q + pos;
+}
+
 __device__ __forceinline__ int wrapInc(int v, int n) { return v + 1 == n ? 0 : v + 1; }
 __device__ __forceinline__ int wrapDec(int v, int n) { return v == 0 ? n - 1 : v - 1; }
 __device__ __forceinline__ int wrapAdd(int v, int c, int n)

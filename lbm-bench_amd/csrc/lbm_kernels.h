@@ -39,6 +39,9 @@ struct StepLaunch {
   float f1f, f2f;
   float scale, invScale;
   int addForce;
+  // launch shape: XCD-aware block order, non-temporal stores (bit 0) /
+  // loads (bit 1), wavefronts per workgroup (4 or 8)
+  int remap, nt, wavesPerBlock;
 };
 
 void launchStepRow(const StepLaunch &L, hipStream_t s);  // lbm_step.hip

@@ -78,6 +78,7 @@ struct LbmDevice {
   float scale, invScale;
   double *dpart, *dout;
   int npart;
+  int remap, nt, wavesPerBlock;
 };
 
 // ---- small helpers ---------------------------------------------------------
@@ -306,6 +307,9 @@ static StepLaunch baseLaunch(const LbmDevice *D, const CaseDataType *cd)
   L.f2f      = 3.0f * (1.0f / 36.0f) * (float)cd->XForce;
   L.scale    = D->scale;
   L.invScale = D->invScale;
+  L.remap         = D->remap;
+  L.nt            = D->nt;
+  L.wavesPerBlock = D->wavesPerBlock;
   return L;
 }
 
@@ -521,9 +525,10 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
   D->segs    = (D->nz + 63) / 64;
   D->ncells  = (uint64_t)D->nx * D->ny * D->nz;
   D->nfluid  = (uint64_t)ld->nFluid;
-  // SoA planes padded to a multiple of 64 elements plus a stagger, so the
-  // 19 streams of a wavefront do not start at the same DRAM channel offset
-  const uint64_t pad = (uint64_t)envInt("LBM_PLANE_PAD", 64);
+  // SoA planes padded to a multiple of 64 elements plus a 64 KiB stagger, so
+  // the 19 streams of a wavefront do not map to the same HBM channels (the
+  // planes are 1 GiB apart at 512^3); measured best of 0 ... 1 MiB
+  const uint64_t pad = (uint64_t)envInt("LBM_PLANE_PAD", (int)(65536 / storageBytes(storageKind(cd))));
   D->plane           = layout == LAYOUT_SOA ? ((D->ncells + 63) / 64) * 64 + pad : D->ncells;
   D->nelem           = layout == LAYOUT_SOA ? D->plane * Q : D->ncells * Q;
   D->narrays         = prop == PROP_AA ? 1 : 2;
@@ -541,6 +546,9 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
   if (D->storage == kNativeStorage)
     D->rhoRef = 0.0;
   D->gsteps  = envInt("LBM_GRAPH_STEPS", 16) & ~1;
+  D->remap   = envInt("LBM_XCD_REMAP", 1);
+  D->nt      = envInt("LBM_NT", 3);
+  D->wavesPerBlock = envInt("LBM_WAVES_PER_BLOCK", 4);
   kd->LoopBalance = 2.0 * Q * (double)D->esize;
 
   HIP_OK(hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking));

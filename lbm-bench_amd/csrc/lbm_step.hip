@@ -23,14 +23,21 @@
 
 namespace lbm {
 
-template <int V, bool AOS, typename S>
-__global__ __launch_bounds__(256) void kStepRow(S *a, S *b, const MaskWin *__restrict__ mask,
-    Params<S> p)
+// minimum wavefronts per SIMD the register allocation must allow
+#ifndef LBM_MIN_WAVES
+#define LBM_MIN_WAVES 1
+#endif
+
+template <int V, bool AOS, int NT, typename S>
+__global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const MaskWin *__restrict__ mask,
+    Params<S> p, int remap)
 {
   using R = typename Store<S>::R;
   using St = Store<S>;
+  constexpr bool NTS = NT & 1, NTL = NT & 2;
 
-  const int wv   = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4u + (threadIdx.x >> 6)));
+  const unsigned blk = remap ? xcdLogicalBlock(blockIdx.x, gridDim.x) : blockIdx.x;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(blk * (blockDim.x >> 6) + (threadIdx.x >> 6)));
   const int lane = (int)(threadIdx.x & 63u);
   const int row  = wv / p.segs;
   if (row >= p.nrows)
@@ -77,12 +84,12 @@ __global__ __launch_bounds__(256) void kStepRow(S *a, S *b, const MaskWin *__res
         else
           i = bb ? slot<AOS>(cell, d, p.plane) : slot<AOS>(cellOf(-cx(d), -cy(d), -cz(d)), inv(d), p.plane);
       }
-      f[d] = St::load(a[i], p.inv);
+      f[d] = St::load(getS<NTL>(&a[i]), p.inv);
     }
   } else {
 #pragma unroll
     for (int d = 0; d < Q; ++d)
-      f[d] = St::load(a[slot<AOS>(cell, d, p.plane)], p.inv);
+      f[d] = St::load(getS<NTL>(&a[slot<AOS>(cell, d, p.plane)]), p.inv);
   }
 
   if (p.addForce)
@@ -93,11 +100,11 @@ __global__ __launch_bounds__(256) void kStepRow(S *a, S *b, const MaskWin *__res
   if constexpr (V == V_PULL) {
 #pragma unroll
     for (int d = 0; d < Q; ++d)
-      b[slot<AOS>(cell, d, p.plane)] = St::store(f[d], p.scale);
+      putS<NTS>(&b[slot<AOS>(cell, d, p.plane)], St::store(f[d], p.scale));
   } else if constexpr (V == V_AA_EVEN) {
 #pragma unroll
     for (int d = 0; d < Q; ++d)
-      a[slot<AOS>(cell, inv(d), p.plane)] = St::store(f[d], p.scale);
+      putS<NTS>(&a[slot<AOS>(cell, inv(d), p.plane)], St::store(f[d], p.scale));
   } else {
     S *o = V == V_PUSH ? b : a;
 #pragma unroll
@@ -109,45 +116,52 @@ __global__ __launch_bounds__(256) void kStepRow(S *a, S *b, const MaskWin *__res
         const bool bb = solidOf(cx(d), cy(d), cz(d));
         i = bb ? slot<AOS>(cell, inv(d), p.plane) : slot<AOS>(cellOf(cx(d), cy(d), cz(d)), d, p.plane);
       }
-      o[i] = St::store(f[d], p.scale);
+      putS<NTS>(&o[i], St::store(f[d], p.scale));
     }
   }
 }
 
-template <int V, bool AOS, typename S>
+template <int V, bool AOS, int NT, typename S>
 static void launchT(const StepLaunch &L, hipStream_t s)
 {
   Params<S> p       = makeParams<S>(L);
+  const int wpb     = L.wavesPerBlock == 8 ? 8 : 4;
   const uint64_t wv = (uint64_t)p.nrows * p.segs;
-  const unsigned blocks = (unsigned)((wv + 3) / 4);
-  hipLaunchKernelGGL((kStepRow<V, AOS, S>), dim3(blocks), dim3(256), 0, s, (S *)L.src,
-      (S *)L.dst, L.g.mask, p);
+  const unsigned blocks = (unsigned)((wv + wpb - 1) / wpb);
+  hipLaunchKernelGGL((kStepRow<V, AOS, NT, S>), dim3(blocks), dim3(64 * wpb), 0, s, (S *)L.src,
+      (S *)L.dst, L.g.mask, p, L.remap);
+}
+
+template <int V, bool AOS, typename S>
+static void launchNT(const StepLaunch &L, hipStream_t s)
+{
+  switch (L.nt & 3) {
+  case 0:
+    return launchT<V, AOS, 0, S>(L, s);
+  case 1:
+    return launchT<V, AOS, 1, S>(L, s);
+  case 2:
+    return launchT<V, AOS, 2, S>(L, s);
+  default:
+    return launchT<V, AOS, 3, S>(L, s);
+  }
 }
 
 template <typename S>
 static void launchS(const StepLaunch &L, hipStream_t s)
 {
-  if (L.g.aos) {
-    switch (L.variant) {
-    case V_PULL:
-      return launchT<V_PULL, true, S>(L, s);
-    case V_PUSH:
-      return launchT<V_PUSH, true, S>(L, s);
-    case V_AA_EVEN:
-      return launchT<V_AA_EVEN, true, S>(L, s);
-    default:
-      return launchT<V_AA_ODD, true, S>(L, s);
-    }
-  }
+  // AoS is the layout of the push/pull variants only (src/kernel_*_aos.c)
+  if (L.g.aos)
+    return L.variant == V_PULL ? launchNT<V_PULL, true, S>(L, s) : launchNT<V_PUSH, true, S>(L, s);
   switch (L.variant) {
   case V_PULL:
-    return launchT<V_PULL, false, S>(L, s);
+    return launchNT<V_PULL, false, S>(L, s);
   case V_PUSH:
-    return launchT<V_PUSH, false, S>(L, s);
+    return launchNT<V_PUSH, false, S>(L, s);
   case V_AA_EVEN:
-    return launchT<V_AA_EVEN, false, S>(L, s);
+    return launchNT<V_AA_EVEN, false, S>(L, s);
   default:
-    return launchT<V_AA_ODD, false, S>(L, s);
+    return launchNT<V_AA_ODD, false, S>(L, s);
   }
 }
 

@@ -25,7 +25,8 @@ I3 = c_int * 3
 
 
 def library_path(precision="dp"):
-    return os.path.join(HERE, f"liblbmhip-{precision}.so")
+    """the in-tree build; LBMHIP_LIBRARY_DIR selects another build directory"""
+    return os.path.join(os.environ.get("LBMHIP_LIBRARY_DIR", HERE), f"liblbmhip-{precision}.so")
 
 
 def driver_path(precision="dp"):
