@@ -22,6 +22,8 @@ import subprocess
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "lbm-bench_amd"))
 
@@ -63,6 +65,46 @@ def cpu_baseline(seconds_hint):
                       f"steps, {float(m.group(2)):.1f} s; the reference cannot index 512^3"}
 
 
+def adjoint_run(lib, lbmhip):
+    """differentiation form: 128^3 porous channel, T = 1024, sqrt(T)
+    checkpoints; FP64 and FP32 (FP32 forward and adjoint)"""
+    dims, T = (128, 128, 128), 1024
+    s = lbmhip.porous_field(lib, dims, 0, 3.0)
+    out = {"workload": "Brinkman TRT 128^3 porous channel (s: U[0,1] noise, Gaussian sigma 3, "
+                       "logistic 4z-2), body force 1e-5, rest start, T 1024, J = outlet-plane "
+                       "kinetic energy, sqrt(T) = 32-step checkpoints"}
+    for form in ("native", "fp32"):
+        A = lbmhip.Adjoint(lib, "channel", dims, (0, 0, 1), omega=1.25, magic=3.0 / 16.0,
+                           xforce=1e-5, storage=form)
+        nf = int(A.fluid_mask().sum())
+        A.set_field(s)
+        A.init("rest", 0, 1.0, 0.0)
+        A.step_times(3)
+        st = A.step_times(20)
+        J, g, stats = A.gradient(T, [32])
+        total = stats["forward_s"] + stats["recompute_s"] + stats["adjoint_s"]
+        key = "fp64" if form == "native" else "fp32"
+        out[key] = {
+            "J": J,
+            "forward_step_ms": round(st["forward_s"] * 1e3, 4),
+            "adjoint_step_ms": round(st["adjoint_s"] * 1e3, 4),
+            "adjoint_to_forward_step": round(st["adjoint_s"] / st["forward_s"], 3),
+            "mlups_forward": round(nf * T / stats["forward_s"] / 1e6, 1),
+            "mlups_adjoint_sweep": round(nf * T / stats["adjoint_s"] / 1e6, 1),
+            "mlups_gradient": round(nf * T / total / 1e6, 1),
+            "gradient_to_forward_time": round(total / stats["forward_s"], 3),
+            "device_gib": round(stats["device_bytes"] / 2 ** 30, 2),
+        }
+        if key == "fp64":
+            g64 = g
+        else:
+            out[key]["gradient_l2_rel_err_vs_fp64"] = float(np.linalg.norm(g - g64) / np.linalg.norm(g64))
+        log(f"adjoint {key}: step ratio {out[key]['adjoint_to_forward_step']}, gradient "
+            f"{out[key]['mlups_gradient']} MLUPS, {out[key]['device_gib']} GiB")
+        A.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +115,7 @@ def main():
     ap.add_argument("--kernel", default="aa-soa")
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -114,30 +157,33 @@ def main():
     log(f"fp64 {args.kernel}: {mlups:.0f} MLUPS, {step_s * 1e3:.3f} ms/step, "
         f"{achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS * 100:.1f} % of 8 TB/s), wall {wall:.2f} s")
 
-    fp16 = None
-    if not args.no_fp16:
-        L16 = lbmhip.Lattice(lib, args.kernel, geo, dims, (0, 0, 1), storage="fp16", **kw)
-        L16.init("random", 0, 1.0, 1e-3)
-        L16.run(args.warmup)
+    forms = {}
+    for form in ([] if args.no_fp16 else ["fp16", "fp32", "bf16"]):
+        Lf = lbmhip.Lattice(lib, args.kernel, geo, dims, (0, 0, 1), storage=form, **kw)
+        Lf.init("random", 0, 1.0, 1e-3)
+        Lf.run(args.warmup)
         lib.synchronize()
-        d16 = L16.run(args.steps)
-        m16 = L16.mass_momentum()
-        cmp = L16.compare_momentum(L64)
-        ml16 = nfluid * args.steps / d16 / 1e6
-        ach16 = BYTES_PER_LUP["fp16"] * nfluid / (d16 / args.steps) / 1e9
-        fp16 = {"mlups": round(ml16, 1), "speedup_vs_fp64": round(ml16 / mlups, 3),
-                "ms_per_step": round(d16 / args.steps * 1e3, 4),
-                "roofline": {"bound": "hbm", "achieved": round(ach16, 1), "peak": HBM_PEAK_GBS,
-                             "unit": "GB/s", "frac": round(ach16 / HBM_PEAK_GBS, 4),
-                             "bytes_per_lup": BYTES_PER_LUP["fp16"]},
-                "velocity_l2_rel_err_vs_fp64": (cmp[0] / cmp[1]) ** 0.5 if cmp[1] > 0 else None,
-                "velocity_max_abs_err_vs_fp64": cmp[2],
-                "mass_drift_rel": (m16[0] - m0[0]) / m0[0],
-                "steps_compared": args.warmup + args.steps}
-        log(f"fp16 {args.kernel}: {ml16:.0f} MLUPS ({ml16 / mlups:.2f}x fp64), "
-            f"u L2 err {fp16['velocity_l2_rel_err_vs_fp64']:.3e}")
-        L16.close()
+        df = Lf.run(args.steps)
+        mf = Lf.mass_momentum()
+        cmp = Lf.compare_momentum(L64)
+        mlf = nfluid * args.steps / df / 1e6
+        achf = BYTES_PER_LUP[form] * nfluid / (df / args.steps) / 1e9
+        forms[form] = {
+            "mlups": round(mlf, 1), "speedup_vs_fp64": round(mlf / mlups, 3),
+            "ms_per_step": round(df / args.steps * 1e3, 4),
+            "roofline": {"bound": "hbm", "achieved": round(achf, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(achf / HBM_PEAK_GBS, 4),
+                         "bytes_per_lup": BYTES_PER_LUP[form]},
+            "velocity_l2_rel_err_vs_fp64": (cmp[0] / cmp[1]) ** 0.5 if cmp[1] > 0 else None,
+            "velocity_max_abs_err_vs_fp64": cmp[2],
+            "mass_drift_rel": (mf[0] - m0[0]) / m0[0],
+            "steps_compared": args.warmup + args.steps}
+        log(f"{form} {args.kernel}: {mlf:.0f} MLUPS ({mlf / mlups:.2f}x fp64), "
+            f"u L2 err {forms[form]['velocity_l2_rel_err_vs_fp64']:.3e}")
+        Lf.close()
     L64.close()
+
+    adjoint = None if args.no_adjoint else adjoint_run(lib, lbmhip)
 
     cpu = None if args.no_cpu else cpu_baseline(20)
 
@@ -163,8 +209,11 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profiles(f"{args.kernel}-fp64-{n}")},
         "cpu_baseline": cpu,
-        "fp16": fp16,
-        "adjoint_mlups": None,
+        "fp16": forms.get("fp16"),
+        "fp32": forms.get("fp32"),
+        "bf16": forms.get("bf16"),
+        "adjoint_mlups": adjoint["fp64"]["mlups_adjoint_sweep"] if adjoint else None,
+        "adjoint": adjoint,
         "mass_drift_rel": (m1[0] - m0[0]) / m0[0],
         "copy_bandwidth_gbs": round(copy_gbs, 1),
         "device": dev,

@@ -62,6 +62,48 @@ int lbmDeviceSynchronize(void);
 /* name and compute-unit count of the current device; 0 on success */
 int lbmDeviceInfo(char *name, int len, int *computeUnits, size_t *globalMem);
 
+/* ---------------------------------------------------------------------
+ * Differentiation form (no counterpart in LBM-Bench).
+ *
+ * A Brinkman-penalised TRT forward model on the same lattice, mask, force
+ * and periodicity rules as the kernels above, and its discrete adjoint.
+ * The collision uses j_s = (1 - s) j in place of j (s = solid fraction per
+ * cell, 0 gives the reference's TRT exactly); the objective is
+ *   J = sum over the fluid cells of the plane x = outletX of 1/2 rho |j|^2
+ * at the final step.  Storage (CaseDataType.Storage): native -> FP64
+ * forward and adjoint; FP32 -> FP32 deviation forward, FP32 adjoint;
+ * FP16 -> FP16 deviation forward, FP32 adjoint.  Gradients accumulate in
+ * FP64.  Arrays: fields are float64 [x][y][z]; states float64 logical
+ * [x][y][z][19] as lbmGetNodes.
+ * ------------------------------------------------------------------- */
+typedef struct LbmAdjoint LbmAdjoint;
+
+LbmAdjoint *lbmAdjointCreate(LatticeDescType *ld, CaseDataType *cd, int outletX);
+void lbmAdjointDestroy(LbmAdjoint *adj);
+void lbmAdjointSetField(LbmAdjoint *adj, const double *s);
+void lbmAdjointSetState(LbmAdjoint *adj, const double *f);
+void lbmAdjointInitState(LbmAdjoint *adj, int mode, uint64_t seed, double rho, double amp);
+/* steps forward from the initial state; returns J of the final state */
+double lbmAdjointForward(LbmAdjoint *adj, int steps, double *seconds);
+/* the state after the last forward run (or the initial state) */
+void lbmAdjointGetState(LbmAdjoint *adj, double *f);
+/* dJ/ds of a `steps`-step run into grad; returns J.  intervals[0..nlevels)
+ * are the checkpoint spacings per level (e.g. {32} for sqrt(T) at
+ * T = 1024; nlevels = 0 keeps every state).  stats = {forward s,
+ * recompute s, adjoint s, device bytes} */
+double lbmAdjointGradient(LbmAdjoint *adj, int steps, const int *intervals, int nlevels,
+    double *grad, double stats[4]);
+/* mean time of one forward step and one adjoint step over `reps` launches */
+void lbmAdjointStepTimes(LbmAdjoint *adj, int reps, double out[2]);
+size_t lbmAdjointDeviceBytes(LbmAdjoint *adj);
+
+/* Porous solid-fraction field: u ~ U[0,1) per cell from the splitmix64
+ * chain over (seed, cell), Gaussian-smoothed with standard deviation sigma
+ * (periodic, radius 3 sigma), standardised to zero mean and unit variance
+ * (z), then s = 1 / (1 + exp(-(a z + b))).  Output float64 [x][y][z]. */
+void lbmPorousField(const int dims[3], uint64_t seed, double sigma, double a, double b,
+    double *s);
+
 #ifdef __cplusplus
 }
 #endif

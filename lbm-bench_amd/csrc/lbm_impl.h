@@ -74,4 +74,71 @@ __device__ __forceinline__ void addForce(R (&f)[Q], R f1, R f2)
   }
 }
 
+// One wavefront owns 64 consecutive z of one (x, y) row.  The row, the
+// segment and the 3x3 neighbour rows are wavefront-uniform (scalar); each
+// lane holds its z and the wrapped z-1 / z+1.  init() returns false for
+// lanes past the lattice end and for solid cells.
+struct RowCtx {
+  const MaskWin *mask;
+  int segs, ny, nz;
+  int lane, row, seg, z;
+  int xs[3], ys[3], zs[3];
+  uint64_t cell;
+
+  __device__ __forceinline__ bool init(const MaskWin *__restrict__ m, int nx, int ny_, int nz_,
+      int segs_, int remap)
+  {
+    mask = m;
+    segs = segs_;
+    ny   = ny_;
+    nz   = nz_;
+    const unsigned blk = remap ? xcdLogicalBlock(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(blk * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+    lane = (int)(threadIdx.x & 63u);
+    row  = wv / segs;
+    if (row >= nx * ny)
+      return false;
+    seg = wv - row * segs;
+    z   = seg * 64 + lane;
+    if (z >= nz)
+      return false;
+    const int x = row / ny, y = row - x * ny;
+    if (bitAt(mask[(uint64_t)row * segs + seg].lo, lane))
+      return false;
+    xs[0] = wrapDec(x, nx);
+    xs[1] = x;
+    xs[2] = wrapInc(x, nx);
+    ys[0] = wrapDec(y, ny);
+    ys[1] = y;
+    ys[2] = wrapInc(y, ny);
+    zs[0] = wrapDec(z, nz);
+    zs[1] = z;
+    zs[2] = wrapInc(z, nz);
+    cell  = (uint64_t)row * nz + z;
+    return true;
+  }
+  // row index of the neighbour row (x+ox, y+oy)
+  __device__ __forceinline__ uint64_t rowOf(int ox, int oy) const
+  {
+    return (uint64_t)xs[ox + 1] * ny + ys[oy + 1];
+  }
+  __device__ __forceinline__ uint64_t cellOf(int ox, int oy, int oz) const
+  {
+    return rowOf(ox, oy) * nz + zs[oz + 1];
+  }
+  __device__ __forceinline__ bool solidOf(int ox, int oy, int oz) const
+  {
+    const MaskWin w  = mask[rowOf(ox, oy) * segs + seg];
+    const RowBits rb = rowBits(w);
+    return bitAt(oz < 0 ? rb.m : (oz > 0 ? rb.p : rb.c), lane);
+  }
+};
+
+// launch geometry of the row kernels
+inline unsigned rowBlocks(const Geom &g, int wavesPerBlock)
+{
+  const uint64_t wv = (uint64_t)g.nx * g.ny * g.segs;
+  return (unsigned)((wv + wavesPerBlock - 1) / wavesPerBlock);
+}
+
 } // namespace lbm

@@ -36,39 +36,12 @@ __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const
   using St = Store<S>;
   constexpr bool NTS = NT & 1, NTL = NT & 2;
 
-  const unsigned blk = remap ? xcdLogicalBlock(blockIdx.x, gridDim.x) : blockIdx.x;
-  const int wv = __builtin_amdgcn_readfirstlane((int)(blk * (blockDim.x >> 6) + (threadIdx.x >> 6)));
-  const int lane = (int)(threadIdx.x & 63u);
-  const int row  = wv / p.segs;
-  if (row >= p.nrows)
+  RowCtx c;
+  if (!c.init(mask, p.nx, p.ny, p.nz, p.segs, remap))
     return;
-  const int seg = wv - row * p.segs;
-  const int z   = seg * 64 + lane;
-  if (z >= p.nz)
-    return;
-  const int x = row / p.ny, y = row - x * p.ny;
-
-  const MaskWin own = mask[(uint64_t)row * p.segs + seg];
-  if (bitAt(own.lo, lane))
-    return;
-
-  const int xs[3] = { wrapDec(x, p.nx), x, wrapInc(x, p.nx) };
-  const int ys[3] = { wrapDec(y, p.ny), y, wrapInc(y, p.ny) };
-  const int zs[3] = { wrapDec(z, p.nz), z, wrapInc(z, p.nz) };
-  const uint64_t cell = (uint64_t)row * p.nz + z;
-
-  // row index of the neighbour row (x+ox, y+oy)
-  auto rowOf = [&](int ox, int oy) -> uint64_t {
-    return (uint64_t)xs[ox + 1] * p.ny + ys[oy + 1];
-  };
-  auto cellOf = [&](int ox, int oy, int oz) -> uint64_t {
-    return rowOf(ox, oy) * p.nz + zs[oz + 1];
-  };
-  auto solidOf = [&](int ox, int oy, int oz) -> bool {
-    const MaskWin w = mask[rowOf(ox, oy) * p.segs + seg];
-    const RowBits rb = rowBits(w);
-    return bitAt(oz < 0 ? rb.m : (oz > 0 ? rb.p : rb.c), lane);
-  };
+  const uint64_t cell = c.cell;
+  auto cellOf  = [&](int ox, int oy, int oz) { return c.cellOf(ox, oy, oz); };
+  auto solidOf = [&](int ox, int oy, int oz) { return c.solidOf(ox, oy, oz); };
 
   R f[Q];
   if constexpr (V == V_PULL || V == V_AA_ODD) {
@@ -126,8 +99,7 @@ static void launchT(const StepLaunch &L, hipStream_t s)
 {
   Params<S> p       = makeParams<S>(L);
   const int wpb     = L.wavesPerBlock == 8 ? 8 : 4;
-  const uint64_t wv = (uint64_t)p.nrows * p.segs;
-  const unsigned blocks = (unsigned)((wv + wpb - 1) / wpb);
+  const unsigned blocks = rowBlocks(L.g, wpb);
   hipLaunchKernelGGL((kStepRow<V, AOS, NT, S>), dim3(blocks), dim3(64 * wpb), 0, s, (S *)L.src,
       (S *)L.dst, L.g.mask, p, L.remap);
 }

@@ -115,6 +115,21 @@ class Library:
         L.kernelSetInitialDensity.argtypes = [LDP, KDP, CDP]
         L.kernelSetInitialVelocity.argtypes = [LDP, KDP, CDP]
         L.kernelAddBodyForce.argtypes = [KDP, LDP, CDP]
+        L.lbmAdjointCreate.argtypes = [LDP, CDP, c_int]
+        L.lbmAdjointCreate.restype = c_void_p
+        L.lbmAdjointDestroy.argtypes = [c_void_p]
+        L.lbmAdjointSetField.argtypes = [c_void_p, dp]
+        L.lbmAdjointSetState.argtypes = [c_void_p, dp]
+        L.lbmAdjointInitState.argtypes = [c_void_p, c_int, ctypes.c_uint64, c_double, c_double]
+        L.lbmAdjointForward.argtypes = [c_void_p, c_int, dp]
+        L.lbmAdjointForward.restype = c_double
+        L.lbmAdjointGetState.argtypes = [c_void_p, dp]
+        L.lbmAdjointGradient.argtypes = [c_void_p, c_int, ctypes.POINTER(c_int), c_int, dp, dp]
+        L.lbmAdjointGradient.restype = c_double
+        L.lbmAdjointStepTimes.argtypes = [c_void_p, c_int, dp]
+        L.lbmAdjointDeviceBytes.argtypes = [c_void_p]
+        L.lbmAdjointDeviceBytes.restype = ctypes.c_size_t
+        L.lbmPorousField.argtypes = [I3, ctypes.c_uint64, c_double, c_double, c_double, dp]
         self.n_kernels = c_int.in_dll(L, "G_N_KERNELS").value
         self.registry = (self.KernelFunctions * self.n_kernels).in_dll(L, "GKernels")
 
@@ -266,6 +281,103 @@ class Lattice:
         if self.kd:
             self.kf.Deinit(ctypes.byref(self.ld), ctypes.byref(self.kd))
             self.kd = ctypes.POINTER(self.lib.KernelData)()
+        if self.ld.Lattice:
+            self.lib.lib.geometryDestroy(ctypes.byref(self.ld))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def porous_field(lib, dims, seed, sigma, a=4.0, b=-2.0):
+    """solid fraction s in (0, 1), float64 (nX, nY, nZ) (lbmPorousField)"""
+    out = np.zeros(tuple(dims))
+    lib.lib.lbmPorousField(I3(*dims), seed, sigma, a, b,
+                           out.ctypes.data_as(ctypes.POINTER(c_double)))
+    return out
+
+
+class Adjoint:
+    """The differentiation form (lbmAdjoint*): Brinkman TRT forward from a
+    fixed initial state, J = sum over x = outlet_x of 1/2 rho |j|^2, and dJ/ds
+    by the hand-written adjoint with multi-level checkpointing."""
+
+    def __init__(self, lib, geometry, dims, periodic=(0, 0, 0), omega=1.0, xforce=0.0,
+                 magic=1.0 / 12.0, storage="native", outlet_x=None, scale=0.0, seed=0):
+        self.lib = lib
+        L = lib.lib
+        self.ld = lib.LatticeDesc()
+        L.geometrySetSeed(seed)
+        L.geometryCreate(geometry.encode(), I3(*dims), I3(*periodic), ctypes.byref(self.ld))
+        self.dims = tuple(dims)
+        self.cd = lib.CaseData()
+        L.caseDataDefaults(ctypes.byref(self.cd))
+        self.cd.Omega = omega
+        self.cd.XForce = xforce
+        self.cd.MagicParam = magic
+        self.cd.Storage = STORAGE[storage]
+        self.cd.StorageScale = scale
+        self.cd.RhoRef = 1.0
+        self.outlet_x = dims[0] - 1 if outlet_x is None else outlet_x
+        self.h = L.lbmAdjointCreate(ctypes.byref(self.ld), ctypes.byref(self.cd), self.outlet_x)
+
+    def fluid_mask(self):
+        n = self.dims[0] * self.dims[1] * self.dims[2]
+        a = np.ctypeslib.as_array(self.ld.Lattice, shape=(n,))
+        return a.reshape(self.dims[2], self.dims[1], self.dims[0]).transpose(2, 1, 0) != 0
+
+    def lattice_flags(self):
+        n = self.dims[0] * self.dims[1] * self.dims[2]
+        return np.ctypeslib.as_array(self.ld.Lattice, shape=(n,)).reshape(
+            self.dims[2], self.dims[1], self.dims[0]).copy()
+
+    def set_field(self, s):
+        s = np.ascontiguousarray(s, dtype=np.float64)
+        assert s.shape == self.dims
+        self.lib.lib.lbmAdjointSetField(self.h, s.ctypes.data_as(ctypes.POINTER(c_double)))
+
+    def set_state(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        self.lib.lib.lbmAdjointSetState(self.h, f.ctypes.data_as(ctypes.POINTER(c_double)))
+
+    def init(self, mode="random", seed=0, rho=1.0, amp=1e-3):
+        self.lib.lib.lbmAdjointInitState(self.h, INIT[mode], seed, rho, amp)
+
+    def forward(self, steps):
+        """-> (J, seconds)"""
+        t = c_double(0.0)
+        J = self.lib.lib.lbmAdjointForward(self.h, steps, ctypes.byref(t))
+        return J, t.value
+
+    def state(self):
+        out = np.empty(self.dims + (19,))
+        self.lib.lib.lbmAdjointGetState(self.h, out.ctypes.data_as(ctypes.POINTER(c_double)))
+        return out
+
+    def gradient(self, steps, intervals=None):
+        """-> (J, dJ/ds (nX, nY, nZ), stats {forward_s, recompute_s, adjoint_s,
+        device_bytes}); intervals default to sqrt(T) checkpoints"""
+        if intervals is None:
+            intervals = [max(1, int(round(steps ** 0.5)))]
+        iv = (c_int * max(1, len(intervals)))(*intervals)
+        grad = np.zeros(self.dims)
+        stats = np.zeros(4)
+        J = self.lib.lib.lbmAdjointGradient(self.h, steps, iv, len(intervals),
+                                            grad.ctypes.data_as(ctypes.POINTER(c_double)),
+                                            stats.ctypes.data_as(ctypes.POINTER(c_double)))
+        return J, grad, dict(zip(("forward_s", "recompute_s", "adjoint_s", "device_bytes"), stats))
+
+    def step_times(self, reps=20):
+        out = np.zeros(2)
+        self.lib.lib.lbmAdjointStepTimes(self.h, reps, out.ctypes.data_as(ctypes.POINTER(c_double)))
+        return {"forward_s": out[0], "adjoint_s": out[1]}
+
+    def close(self):
+        if self.h:
+            self.lib.lib.lbmAdjointDestroy(self.h)
+            self.h = None
         if self.ld.Lattice:
             self.lib.lib.geometryDestroy(ctypes.byref(self.ld))
 

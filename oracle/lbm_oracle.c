@@ -716,3 +716,272 @@ double oracle_verify(const int dims[3], const int *lat, const double *pdfs, doub
   free(prof);
   return sqrt(dev);
 }
+
+/* --------------------------------------------------------------------- */
+/* differentiation form: Brinkman-penalised TRT and its adjoint           */
+/*                                                                        */
+/* Not in the reference.  The forward model is the push step above on a   */
+/* halo-free lattice (periodic wrap, half-way bounce-back off obstacles,  */
+/* force 3 w_d c_dx g on the streamed value) with j_s = (1 - s) j in the   */
+/* equilibrium.  The adjoint transposes the streaming permutation and     */
+/* obtains the per-cell Jacobian of the collision by forward-mode         */
+/* automatic differentiation (dual numbers), so it shares no hand-derived */
+/* formula with the GPU kernel it checks.                                 */
+/* --------------------------------------------------------------------- */
+typedef struct {
+  double v, d;
+} dual;
+
+static inline dual dd(double v) { dual r = { v, 0.0 }; return r; }
+static inline dual dadd(dual a, dual b) { dual r = { a.v + b.v, a.d + b.d }; return r; }
+static inline dual dsub(dual a, dual b) { dual r = { a.v - b.v, a.d - b.d }; return r; }
+static inline dual dmul(dual a, dual b) { dual r = { a.v * b.v, a.d * b.v + a.v * b.d }; return r; }
+static inline dual dscale(dual a, double c) { dual r = { a.v * c, a.d * c }; return r; }
+static inline dual dneg(dual a) { dual r = { -a.v, -a.d }; return r; }
+
+static dual dsum5(const dual *f, const int *plus, const int *minus)
+{
+  dual s = f[plus[0]];
+  for (int k = 1; k < 5; ++k)
+    s = dadd(s, f[plus[k]]);
+  for (int k = 0; k < 5; ++k)
+    s = dsub(s, f[minus[k]]);
+  return s;
+}
+
+/* the TRT update of trt_collide with j_s = keep * j, in dual numbers */
+static void brinkman_collide(const dual *f, dual *o, const trt_t *p, dual keep)
+{
+  const dual ux = dsum5(f, JX_PLUS, JX_MINUS), uy = dsum5(f, JY_PLUS, JY_MINUS),
+             uz = dsum5(f, JZ_PLUS, JZ_MINUS);
+  dual rho = f[RHO_ORDER[0]];
+  for (int k = 1; k < Q; ++k)
+    rho = dadd(rho, f[RHO_ORDER[k]]);
+  const dual sx = dmul(keep, ux), sy = dmul(keep, uy), sz = dmul(keep, uz);
+  const dual q  = dadd(dadd(dmul(sx, sx), dmul(sy, sy)), dmul(sz, sz));
+  const dual D  = dsub(rho, dscale(dscale(q, 3.0), 0.5));
+  o[C] = dsub(f[C], dscale(dsub(f[C], dscale(D, p->w0)), p->we));
+  const dual s3[3] = { sx, sy, sz };
+  for (int k = 0; k < 9; ++k) {
+    const int i = PAIR[k][0], j = PAIR[k][1];
+    const int ci[3] = { CX[i], CY[i], CZ[i] };
+    dual ui = dd(0.0);
+    int first = 1;
+    for (int a = 0; a < 3; ++a) {
+      if (ci[a] == 0)
+        continue;
+      const dual t = ci[a] > 0 ? s3[a] : dneg(s3[a]);
+      ui           = first ? t : dadd(ui, t);
+      first        = 0;
+    }
+    const int axis  = k < 3;
+    const double wX3 = axis ? p->w1x3 : p->w2x3, wNH = axis ? p->w1nh : p->w2nh;
+    const dual wD   = dscale(D, axis ? p->w1 : p->w2);
+    const dual even = dscale(dsub(dsub(dscale(dadd(f[i], f[j]), 0.5), dscale(dmul(ui, ui), wNH)), wD), p->we);
+    const dual odd  = dscale(dsub(dscale(dsub(f[i], f[j]), 0.5), dscale(ui, wX3)), p->wo);
+    o[i] = dsub(dsub(f[i], even), odd);
+    o[j] = dadd(dsub(f[j], even), odd);
+  }
+}
+
+typedef struct {
+  int n[3];
+  const int *lat;
+  const double *s;
+  trt_t p;
+  double fd[Q];
+} bk_t;
+
+static inline int bk_wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+static inline int64_t bk_cell(const bk_t *b, int x, int y, int z)
+{
+  return ((int64_t)x * b->n[1] + y) * b->n[2] + z;
+}
+static inline int bk_fluid(const bk_t *b, int x, int y, int z)
+{
+  return b->lat[lat_index(b->n, x, y, z)] != OBST;
+}
+
+static void bk_step(const bk_t *b, const double *in, double *out)
+{
+  const int nx = b->n[0], ny = b->n[1], nz = b->n[2];
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y)
+      for (int z = 0; z < nz; ++z) {
+        if (!bk_fluid(b, x, y, z))
+          continue;
+        const int64_t c = bk_cell(b, x, y, z);
+        dual f[Q], g[Q];
+        for (int d = 0; d < Q; ++d)
+          f[d] = dd(in[c * Q + d]);
+        brinkman_collide(f, g, &b->p, dd(1.0 - b->s[c]));
+        for (int d = 0; d < Q; ++d) {
+          const int tx = bk_wrap(x + CX[d], nx), ty = bk_wrap(y + CY[d], ny), tz = bk_wrap(z + CZ[d], nz);
+          if (bk_fluid(b, tx, ty, tz))
+            out[bk_cell(b, tx, ty, tz) * Q + d] = g[d].v + b->fd[d];
+          else
+            out[c * Q + INV[d]] = g[d].v + b->fd[INV[d]];
+        }
+      }
+}
+
+static double bk_objective(const bk_t *b, const double *f, int outletX, double *lam)
+{
+  const int nx = b->n[0], ny = b->n[1], nz = b->n[2];
+  double J = 0.0;
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y)
+      for (int z = 0; z < nz; ++z) {
+        const int64_t c = bk_cell(b, x, y, z);
+        double rho = 0.0, j[3] = { 0, 0, 0 };
+        const int on = x == outletX && bk_fluid(b, x, y, z);
+        if (on)
+          for (int d = 0; d < Q; ++d) {
+            rho += f[c * Q + d];
+            j[0] += CX[d] * f[c * Q + d];
+            j[1] += CY[d] * f[c * Q + d];
+            j[2] += CZ[d] * f[c * Q + d];
+          }
+        const double j2 = j[0] * j[0] + j[1] * j[1] + j[2] * j[2];
+        if (on)
+          J += 0.5 * rho * j2;
+        if (lam)
+          for (int d = 0; d < Q; ++d)
+            lam[c * Q + d] = on ? 0.5 * j2 + rho * (CX[d] * j[0] + CY[d] * j[1] + CZ[d] * j[2]) : 0.0;
+      }
+  return J;
+}
+
+/* T steps of the Brinkman model from init (logical [x][y][z][19]) with
+ * solid fraction s ([x][y][z]); returns J at x = outletX.  final_state and
+ * grad (dJ/ds, [x][y][z]) may be NULL. */
+double oracle_brinkman(const int dims[3], const int *lattice, const double *s,
+    const double *init, int T, double omega, double magic, double xforce, int outletX,
+    double *final_state, double *grad)
+{
+  bk_t b;
+  for (int a = 0; a < 3; ++a)
+    b.n[a] = dims[a];
+  b.lat = lattice;
+  b.s   = s;
+  trt_setup(&b.p, omega, magic);
+  for (int d = 0; d < Q; ++d)
+    b.fd[d] = 3.0 * WEIGHT[d] * CX[d] * xforce;
+  const int64_t nc = (int64_t)dims[0] * dims[1] * dims[2];
+  const size_t sz  = (size_t)(nc * Q);
+  const int keep_all = grad != NULL;
+  double *states = (double *)calloc(sz * (size_t)(keep_all ? T + 1 : 2), sizeof(double));
+  memcpy(states, init, sz * sizeof(double));
+  for (int t = 0; t < T; ++t) {
+    double *in  = states + sz * (size_t)(keep_all ? t : (t & 1));
+    double *out = states + sz * (size_t)(keep_all ? t + 1 : ((t + 1) & 1));
+    memcpy(out, in, sz * sizeof(double)); /* obstacle slots keep their values */
+    bk_step(&b, in, out);
+  }
+  const double *fT = states + sz * (size_t)(keep_all ? T : (T & 1));
+  if (final_state)
+    memcpy(final_state, fT, sz * sizeof(double));
+  double *lam = grad ? (double *)calloc(sz, sizeof(double)) : NULL;
+  const double J = bk_objective(&b, fT, outletX, lam);
+  if (grad) {
+    double *lam2 = (double *)calloc(sz, sizeof(double));
+    for (int64_t i = 0; i < nc; ++i)
+      grad[i] = 0.0;
+    const int nx = dims[0], ny = dims[1], nz = dims[2];
+    for (int t = T - 1; t >= 0; --t) {
+      const double *ft = states + sz * (size_t)t;
+#pragma omp parallel for collapse(2) schedule(static)
+      for (int x = 0; x < nx; ++x)
+        for (int y = 0; y < ny; ++y)
+          for (int z = 0; z < nz; ++z) {
+            if (!bk_fluid(&b, x, y, z))
+              continue;
+            const int64_t c = bk_cell(&b, x, y, z);
+            double mu[Q];
+            /* transpose of the streaming permutation */
+            for (int k = 0; k < Q; ++k) {
+              const int tx = bk_wrap(x + CX[k], nx), ty = bk_wrap(y + CY[k], ny), tz = bk_wrap(z + CZ[k], nz);
+              mu[k] = bk_fluid(&b, tx, ty, tz) ? lam[bk_cell(&b, tx, ty, tz) * Q + k] : lam[c * Q + INV[k]];
+            }
+            dual f[Q], g[Q];
+            for (int d = 0; d < Q; ++d)
+              f[d] = dd(ft[c * Q + d]);
+            /* column d of the collision Jacobian: seed f_d */
+            for (int d = 0; d < Q; ++d) {
+              f[d].d = 1.0;
+              brinkman_collide(f, g, &b.p, dd(1.0 - s[c]));
+              f[d].d = 0.0;
+              double acc = 0.0;
+              for (int k = 0; k < Q; ++k)
+                acc += mu[k] * g[k].d;
+              lam2[c * Q + d] = acc;
+            }
+            /* d g / d s: seed keep = 1 - s with derivative -1 */
+            dual keep = { 1.0 - s[c], -1.0 };
+            brinkman_collide(f, g, &b.p, keep);
+            double acc = 0.0;
+            for (int k = 0; k < Q; ++k)
+              acc += mu[k] * g[k].d;
+            grad[c] += acc;
+          }
+      double *tmp = lam;
+      lam         = lam2;
+      lam2        = tmp;
+    }
+    free(lam2);
+    free(lam);
+  }
+  free(states);
+  return J;
+}
+
+/* the porous field of the library, restated: u from the hash chain over
+ * (seed, cell, 0), periodic Gaussian smoothing (radius ceil(3 sigma)),
+ * standardised, then the logistic a z + b */
+void oracle_porous_field(const int dims[3], uint64_t seed, double sigma, double a, double b,
+    double *s)
+{
+  const int nx = dims[0], ny = dims[1], nz = dims[2];
+  const int64_t n = (int64_t)nx * ny * nz;
+  double *u = (double *)malloc(sizeof(double) * (size_t)n);
+  for (int64_t i = 0; i < n; ++i)
+    u[i] = u01(oracle_hash(seed, (uint64_t)i, 0));
+  const int rad = sigma > 0.0 ? (int)ceil(3.0 * sigma) : 0;
+  double *w = (double *)malloc(sizeof(double) * (size_t)(2 * rad + 1));
+  double wsum = 0.0;
+  for (int k = -rad; k <= rad; ++k) {
+    w[k + rad] = sigma > 0.0 ? exp(-0.5 * (double)k * k / (sigma * sigma)) : 1.0;
+    wsum += w[k + rad];
+  }
+  for (int k = 0; k <= 2 * rad; ++k)
+    w[k] /= wsum;
+  double *t = (double *)malloc(sizeof(double) * (size_t)n);
+  const int nn[3] = { nx, ny, nz };
+  for (int axis = 0; axis < 3; ++axis) {
+    for (int x = 0; x < nx; ++x)
+      for (int y = 0; y < ny; ++y)
+        for (int z = 0; z < nz; ++z) {
+          double acc = 0.0;
+          for (int k = -rad; k <= rad; ++k) {
+            int p[3] = { x, y, z };
+            p[axis]  = ((p[axis] + k) % nn[axis] + nn[axis]) % nn[axis];
+            acc += w[k + rad] * u[((int64_t)p[0] * ny + p[1]) * nz + p[2]];
+          }
+          t[((int64_t)x * ny + y) * nz + z] = acc;
+        }
+    memcpy(u, t, sizeof(double) * (size_t)n);
+  }
+  double mean = 0.0, var = 0.0;
+  for (int64_t i = 0; i < n; ++i)
+    mean += u[i];
+  mean /= (double)n;
+  for (int64_t i = 0; i < n; ++i)
+    var += (u[i] - mean) * (u[i] - mean);
+  const double sd = sqrt(var / (double)n);
+  for (int64_t i = 0; i < n; ++i)
+    s[i] = 1.0 / (1.0 + exp(-(a * (sd > 0.0 ? (u[i] - mean) / sd : 0.0) + b)));
+  free(u);
+  free(t);
+  free(w);
+}

@@ -37,6 +37,16 @@ double oracle_verify(const int dims[3], const int *lattice, const double *pdfs,
 void oracle_moments(const int dims[3], const int *lattice, const double *pdfs,
     double out[4]);
 
+/* differentiation form: Brinkman TRT forward, J = sum over x = outletX of
+ * 1/2 rho |j|^2, and dJ/ds by the discrete adjoint (grad may be NULL) */
+double oracle_brinkman(const int dims[3], const int *lattice, const double *s,
+    const double *init, int T, double omega, double magic, double xforce, int outletX,
+    double *final_state, double *grad);
+
+/* porous solid-fraction field (smoothed, standardised, logistic) */
+void oracle_porous_field(const int dims[3], uint64_t seed, double sigma, double a, double b,
+    double *s);
+
 /* the hash used for every seeded input */
 uint64_t oracle_hash(uint64_t seed, uint64_t a, uint64_t b);
 

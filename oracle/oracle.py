@@ -57,6 +57,12 @@ class Oracle:
         L.oracle_moments.restype = None
         L.oracle_hash.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
         L.oracle_hash.restype = ctypes.c_uint64
+        L.oracle_brinkman.argtypes = [_I3, _ip, _dp, _dp, ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp]
+        L.oracle_brinkman.restype = ctypes.c_double
+        L.oracle_porous_field.argtypes = [_I3, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_double, _dp]
+        L.oracle_porous_field.restype = None
 
     def geometry(self, gtype, dims, periodic=(0, 0, 0), seed=0):
         """-> (lattice[nZ,nY,nX] int32, periodic tuple, (nFluid, nObst))"""
@@ -115,6 +121,24 @@ class Oracle:
 
     def hash(self, seed, a, b):
         return self.lib.oracle_hash(seed, a, b)
+
+    def brinkman(self, dims, lattice, s, init, steps, omega, magic, xforce, outlet_x,
+                 gradient=False):
+        """-> (J, final state, dJ/ds or None) of the Brinkman TRT model"""
+        lat = np.ascontiguousarray(lattice, dtype=np.int32)
+        s = np.ascontiguousarray(s, dtype=np.float64)
+        init = np.ascontiguousarray(init, dtype=np.float64)
+        final = np.zeros(tuple(dims) + (19,))
+        grad = np.zeros(tuple(dims)) if gradient else None
+        J = self.lib.oracle_brinkman(_I3(*dims), _ptr(lat, _ip), _ptr(s, _dp), _ptr(init, _dp),
+                                     steps, omega, magic, xforce, outlet_x, _ptr(final, _dp),
+                                     _ptr(grad, _dp) if gradient else None)
+        return J, final, grad
+
+    def porous_field(self, dims, seed, sigma, a=4.0, b=-2.0):
+        out = np.zeros(tuple(dims))
+        self.lib.oracle_porous_field(_I3(*dims), seed, sigma, a, b, _ptr(out, _dp))
+        return out
 
 
 class Reference:

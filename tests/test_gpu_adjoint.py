@@ -1,0 +1,156 @@
+"""The differentiation form on the GPU (lbmAdjoint* through the C-ABI):
+  - the Brinkman forward step equals the oracle (and at s = 0 the
+    reference's push step) to 1e-12;
+  - the adjoint gradient equals the oracle's FP64 adjoint (dual-number
+    Jacobians) for every checkpoint layout;
+  - 128^3 porous channel, T = 1024, sqrt(T) checkpoints: dJ/ds matches
+    central finite differences (h = 1e-6) on 64 seeded cells to 1e-4;
+  - a torch.autograd.Function around the C-ABI passes gradcheck;
+  - an adjoint step costs at most 2.5 forward steps."""
+import numpy as np
+import pytest
+
+from lbmhip import Adjoint, porous_field
+
+pytestmark = pytest.mark.gpu
+
+KW = dict(omega=1.25, magic=3 / 16)
+
+
+def _field(oracle, dims, seed, sigma=1.0, scale=0.6):
+    return oracle.porous_field(dims, seed, sigma) * scale
+
+
+def test_porous_field_matches_oracle(gpu_lib, oracle):
+    for dims, sigma in (((16, 12, 20), 3.0), ((9, 7, 5), 0.0)):
+        got = porous_field(gpu_lib, dims, 11, sigma)
+        assert np.abs(got - oracle.porous_field(dims, 11, sigma)).max() < 1e-13
+
+
+@pytest.mark.parametrize("dims", [(8, 10, 12), (6, 9, 70)])
+def test_forward_matches_oracle(gpu_lib, oracle, dims):
+    lat, per, _ = oracle.geometry("channel", dims, (0, 0, 1))
+    init = oracle.init_random(dims, lat, 5, 2e-2)
+    fluid = lat.transpose(2, 1, 0) != 0
+    for s in (np.zeros(dims), _field(oracle, dims, 2)):
+        with Adjoint(gpu_lib, "channel", dims, (0, 0, 1), xforce=1e-4, **KW) as A:
+            A.set_field(s)
+            A.set_state(init)
+            J, _ = A.forward(9)
+            got = A.state()
+        Jo, exp, _ = oracle.brinkman(dims, lat, s, init, 9, 1.25, 3 / 16, 1e-4, dims[0] - 1)
+        assert np.abs(got - exp)[fluid].max() < 1e-12 * np.abs(exp[fluid]).max()
+        assert abs(J - Jo) <= 1e-12 * abs(Jo)
+        if not s.any():
+            push, _ = oracle.run("push-soa", dims, lat, per, init, 9, 1.25, 3 / 16, 1e-4, True)
+            assert np.abs(got - push)[fluid].max() < 1e-12 * np.abs(push[fluid]).max()
+
+
+@pytest.mark.parametrize("intervals", [[], [5], [7, 2], [24], [1]])
+def test_gradient_matches_oracle(gpu_lib, oracle, intervals):
+    dims, T = (8, 10, 70), 24
+    lat, per, _ = oracle.geometry("channel", dims, (0, 0, 1))
+    init = oracle.init_random(dims, lat, 9, 2e-2)
+    s = _field(oracle, dims, 4)
+    Jo, _, go = oracle.brinkman(dims, lat, s, init, T, 1.25, 3 / 16, 1e-3, 5, gradient=True)
+    with Adjoint(gpu_lib, "channel", dims, (0, 0, 1), xforce=1e-3, outlet_x=5, **KW) as A:
+        A.set_field(s)
+        A.set_state(init)
+        J, g, stats = A.gradient(T, intervals)
+    assert abs(J - Jo) <= 1e-12 * abs(Jo)
+    assert np.abs(g - go).max() <= 1e-10 * np.abs(go).max()
+
+
+@pytest.mark.parametrize("storage,bound", [("fp32", 1e-3), ("fp16", 2e-2)])
+def test_low_precision_gradient(gpu_lib, oracle, storage, bound):
+    """FP32 / FP16 forward with an FP32 adjoint against the FP64 oracle"""
+    dims, T = (8, 10, 64), 24
+    lat, per, _ = oracle.geometry("channel", dims, (0, 0, 1))
+    s = _field(oracle, dims, 4)
+    init = oracle.init_random(dims, lat, 9, 2e-2)
+    _, _, go = oracle.brinkman(dims, lat, s, init, T, 1.25, 3 / 16, 1e-3, 5, gradient=True)
+    with Adjoint(gpu_lib, "channel", dims, (0, 0, 1), xforce=1e-3, outlet_x=5, storage=storage,
+                 **KW) as A:
+        A.set_field(s)
+        A.set_state(init)
+        _, g, _ = A.gradient(T, [5])
+    err = np.linalg.norm(g - go) / np.linalg.norm(go)
+    print(f"{storage}: gradient L2 error vs FP64 {err:.3e}")
+    assert err <= bound
+
+
+def test_porous_channel_128_finite_differences(gpu_lib):
+    """128^3 porous channel (walls at y, periodic x and z), s from U[0,1]
+    noise smoothed with sigma = 3 and a logistic, body force 1e-5, rest
+    start, T = 1024, J at the outlet plane x = 127; sqrt(T) checkpoints"""
+    dims, T, h = (128, 128, 128), 1024, 1e-6
+    s = porous_field(gpu_lib, dims, 0, 3.0)
+    with Adjoint(gpu_lib, "channel", dims, (0, 0, 1), xforce=1e-5, **KW) as A:
+        fluid = A.fluid_mask()
+        A.set_field(s)
+        A.init("rest", 0, 1.0, 0.0)
+        J, g, stats = A.gradient(T, [32])
+        J0, _ = A.forward(T)
+        assert abs(J0 - J) <= 1e-13 * abs(J)
+        rng = np.random.default_rng(64)
+        cells = np.argwhere(fluid)[rng.choice(int(fluid.sum()), 64, replace=False)]
+        floor = 1e-2 * np.abs(g).max()
+        errs = []
+        for c in map(tuple, cells):
+            keep = s[c]
+            s[c] = keep + h
+            A.set_field(s)
+            Jp, _ = A.forward(T)
+            s[c] = keep - h
+            A.set_field(s)
+            Jm, _ = A.forward(T)
+            s[c] = keep
+            fd = (Jp - Jm) / (2 * h)
+            errs.append(abs(fd - g[c]) / max(abs(g[c]), floor))
+        A.set_field(s)
+    print(f"J = {J:.6e}; FD relative error: max {max(errs):.2e}, median {np.median(errs):.2e}; "
+          f"forward {stats['forward_s']:.3f} s, recompute {stats['recompute_s']:.3f} s, "
+          f"adjoint {stats['adjoint_s']:.3f} s, device {stats['device_bytes'] / 2**30:.1f} GiB")
+    assert max(errs) <= 1e-4
+
+
+def test_autograd_function_gradcheck(gpu_lib):
+    import torch
+
+    dims, T = (6, 8, 6), 10
+    A = Adjoint(gpu_lib, "channel", dims, (0, 0, 1), xforce=1e-3, outlet_x=3, **KW)
+    A.init("random", 1, 1.0, 2e-2)
+
+    class OutletEnergy(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, s):
+            ctx.save_for_backward(s)
+            A.set_field(s.detach().numpy())
+            J, _ = A.forward(T)
+            return torch.tensor(J, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, gJ):
+            (s,) = ctx.saved_tensors
+            A.set_field(s.detach().numpy())
+            _, g, _ = A.gradient(T, [3])
+            return gJ * torch.from_numpy(g)
+
+    try:
+        s = torch.from_numpy(np.full(dims, 0.3)).requires_grad_(True)
+        assert torch.autograd.gradcheck(OutletEnergy.apply, (s,), eps=1e-6, atol=1e-11, rtol=1e-5)
+    finally:
+        A.close()
+
+
+def test_adjoint_step_cost(gpu_lib):
+    dims = (256, 256, 256)
+    with Adjoint(gpu_lib, "channel", dims, (0, 0, 1), xforce=1e-5, **KW) as A:
+        A.init("rest", 0, 1.0, 0.0)
+        A.set_field(np.full(dims, 0.1))
+        A.step_times(3)
+        t = A.step_times(20)
+    ratio = t["adjoint_s"] / t["forward_s"]
+    print(f"forward step {t['forward_s'] * 1e3:.3f} ms, adjoint step {t['adjoint_s'] * 1e3:.3f} ms, "
+          f"ratio {ratio:.2f}")
+    assert ratio <= 2.5
