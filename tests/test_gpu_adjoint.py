@@ -82,8 +82,12 @@ def test_low_precision_gradient(gpu_lib, oracle, storage, bound):
 def test_porous_channel_128_finite_differences(gpu_lib):
     """128^3 porous channel (walls at y, periodic x and z), s from U[0,1]
     noise smoothed with sigma = 3 and a logistic, body force 1e-5, rest
-    start, T = 1024, J at the outlet plane x = 127; sqrt(T) checkpoints"""
-    dims, T, h = (128, 128, 128), 1024, 1e-6
+    start, T = 1024, J at the outlet plane x = 127; sqrt(T) checkpoints.
+    At h = 1e-6 the central difference carries the round-off of two
+    1024-step FP64 runs (about 1e-17 of J), so its per-cell error is
+    checked by the median; at h = 1e-4 that noise is 100x smaller and the
+    O(h^2) truncation still far below 1e-4, so every cell is checked."""
+    dims, T = (128, 128, 128), 1024
     s = porous_field(gpu_lib, dims, 0, 3.0)
     with Adjoint(gpu_lib, "channel", dims, (0, 0, 1), xforce=1e-5, **KW) as A:
         fluid = A.fluid_mask()
@@ -95,23 +99,27 @@ def test_porous_channel_128_finite_differences(gpu_lib):
         rng = np.random.default_rng(64)
         cells = np.argwhere(fluid)[rng.choice(int(fluid.sum()), 64, replace=False)]
         floor = 1e-2 * np.abs(g).max()
-        errs = []
-        for c in map(tuple, cells):
-            keep = s[c]
-            s[c] = keep + h
-            A.set_field(s)
-            Jp, _ = A.forward(T)
-            s[c] = keep - h
-            A.set_field(s)
-            Jm, _ = A.forward(T)
-            s[c] = keep
-            fd = (Jp - Jm) / (2 * h)
-            errs.append(abs(fd - g[c]) / max(abs(g[c]), floor))
+        errs = {}
+        for h in (1e-6, 1e-4):
+            errs[h] = []
+            for c in map(tuple, cells):
+                keep = s[c]
+                s[c] = keep + h
+                A.set_field(s)
+                Jp, _ = A.forward(T)
+                s[c] = keep - h
+                A.set_field(s)
+                Jm, _ = A.forward(T)
+                s[c] = keep
+                fd = (Jp - Jm) / (2 * h)
+                errs[h].append(abs(fd - g[c]) / max(abs(g[c]), floor))
         A.set_field(s)
-    print(f"J = {J:.6e}; FD relative error: max {max(errs):.2e}, median {np.median(errs):.2e}; "
-          f"forward {stats['forward_s']:.3f} s, recompute {stats['recompute_s']:.3f} s, "
+    for h, e in errs.items():
+        print(f"h = {h:g}: FD relative error max {max(e):.2e}, median {np.median(e):.2e}")
+    print(f"J = {J:.6e}; forward {stats['forward_s']:.3f} s, recompute {stats['recompute_s']:.3f} s, "
           f"adjoint {stats['adjoint_s']:.3f} s, device {stats['device_bytes'] / 2**30:.1f} GiB")
-    assert max(errs) <= 1e-4
+    assert np.median(errs[1e-6]) <= 1e-4
+    assert max(errs[1e-4]) <= 1e-4
 
 
 def test_autograd_function_gradcheck(gpu_lib):

@@ -1,0 +1,47 @@
+"""The benchmark driver binary on the GPU: LBM-Bench's -V verification run
+(src/test.sh:61-103) for every kernel in both builds, and a benchmark run
+whose output lines keep the reference's format (src/main.c:154-171)."""
+import json
+import os
+import re
+import subprocess
+
+import pytest
+from conftest import GOLDEN
+
+import lbmhip
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(precision, *args):
+    return subprocess.run([lbmhip.driver_path(precision), *args], capture_output=True, text=True,
+                          timeout=300)
+
+
+@pytest.mark.parametrize("precision", ["dp", "sp"])
+def test_verification_runs(precision):
+    with open(os.path.join(GOLDEN, "verification.json")) as f:
+        ver = json.load(f)[precision]
+    spread = max(ver.values()) - min(ver.values())
+    for kernel in lbmhip.KERNELS:
+        r = _run(precision, "-V", "-k", kernel)
+        assert r.returncode == 0, r.stdout[-2000:]
+        assert "VERIFICATION SUCCEEDED" in r.stdout
+        norm = float(re.search(r"deviation from analytical solution: (\S+)", r.stdout).group(1))
+        tol = 1e-6 * ver[kernel] if precision == "dp" else 2 * spread
+        assert abs(norm - ver[kernel]) <= tol, (kernel, norm, ver[kernel])
+
+
+def test_benchmark_output_format():
+    r = _run("dp", "-d", "64x64x64", "-g", "channel", "-z", "-k", "aa-soa", "-i", "50", "-o",
+             "1.25", "-m", "0.1875", "-B", "-f", "1e-6", "-I", "random", "-s", "3")
+    assert r.returncode == 0, r.stdout[-2000:]
+    m = re.search(r"^P:\s+([0-9.]+) MFLUP/s\s+d: ([0-9.]+) s\s+iter: 50\s+fnodes: ([0-9.]+) x1e6"
+                  r"\s+geo: channel\s+kernel: aa-soa$", r.stdout, flags=re.M)
+    assert m and float(m.group(1)) > 0
+    assert abs(float(m.group(3)) - 64 * 62 * 64 / 1e6) < 1e-6
+    drift = float(re.search(r"mass drift:\s+(\S+)", r.stdout).group(1))
+    assert abs(drift) < 1e-13
+    for line in ("# Evaluation Stats", "#   runtime:", "#   MEM bandwidth:", "#   performance:"):
+        assert line in r.stdout
