@@ -126,6 +126,8 @@ typedef struct LatticeDesc {
   const char *Name;
 } LatticeDescType;
 
+#define LBM_CASE_EXTENSIONS 0x4C424D31 /* "LBM1" */
+
 typedef struct CaseData {
   PdfType Omega;
   PdfType RhoIn;
@@ -134,6 +136,11 @@ typedef struct CaseData {
   int MaxIterations;
   int StatisticsModulus;
   /* --- appended by this implementation --- */
+  int Extensions;      /* LBM_CASE_EXTENSIONS when the fields below are set
+                          (caseDataDefaults); otherwise the library uses the
+                          reference's behaviour: Lambda 1/12, native storage,
+                          body force from the LBM_BODY_FORCE environment
+                          variable (the reference's -DVERIFICATION switch) */
   double MagicParam;   /* TRT Lambda; the reference's value is 1/12       */
   int BodyForce;       /* 1: add 3 w_i c_ix XForce after every step        */
   int Storage;         /* LBM_STORE_*                                       */
@@ -190,8 +197,14 @@ static inline int latticeIndex(int dims[3], int x, int y, int z)
 #endif
 
 /* Sets CaseDataType's appended fields to the reference's behaviour:
- * Lambda = 1/12, no body force, native storage, RhoRef = 1. */
+ * Lambda = 1/12, no body force, native storage, RhoRef = 1, and marks them
+ * valid.  A driver that never calls it (LBM-Bench's own main.c) gets the
+ * same behaviour, with the body force taken from LBM_BODY_FORCE. */
 void caseDataDefaults(CaseDataType *cd);
+
+/* The case as the kernels use it: cd itself when its appended fields are
+ * marked valid, else cd's reference fields plus the defaults above. */
+CaseDataType caseDataEffective(const CaseDataType *cd);
 
 /* Geometry types: box, channel, pipe, blocks[-N], fluid (src/lbm.c:49-156)
  * and spheres[-N] (N seeded random spheres inside a channel, radius

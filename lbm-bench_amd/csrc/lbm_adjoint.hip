@@ -570,7 +570,8 @@ LbmAdjoint *lbmAdjointCreate(LatticeDescType *ld, CaseDataType *cd, int outletX)
   if (outletX < 0 || outletX >= nx)
     adjFail("lbmAdjointCreate: outlet plane outside the lattice");
   LbmAdjoint *A = new LbmAdjoint();
-  A->cd         = *cd;
+  A->cd         = caseDataEffective(cd);
+  cd            = &A->cd;
   A->outletX    = outletX;
   const char *rm = getenv("LBM_XCD_REMAP");
   A->remap      = rm && *rm ? atoi(rm) : 1;

@@ -37,11 +37,23 @@ const int G_N_KERNELS = sizeof(GKernels) / sizeof(GKernels[0]);
 
 void caseDataDefaults(CaseDataType *cd)
 {
+  cd->Extensions   = LBM_CASE_EXTENSIONS;
   cd->MagicParam   = 1.0 / 12.0;
   cd->BodyForce    = 0;
   cd->Storage      = LBM_STORE_NATIVE;
   cd->StorageScale = 0.0;
   cd->RhoRef       = 1.0;
+}
+
+CaseDataType caseDataEffective(const CaseDataType *cd)
+{
+  CaseDataType e = *cd;
+  if (cd->Extensions != LBM_CASE_EXTENSIONS) {
+    caseDataDefaults(&e);
+    const char *bf = getenv("LBM_BODY_FORCE");
+    e.BodyForce    = bf != NULL && atoi(bf) != 0;
+  }
+  return e;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -339,8 +351,10 @@ void kernelStatistics(KernelDataType *kd, LatticeDescType *ld, CaseDataType *cd,
     printf("# iter: %4d   avg density: %e\n", iter, (double)kernelDensity(kd, ld));
 }
 
-void kernelVerify(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd, PdfType *errorNorm)
+void kernelVerify(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cdIn, PdfType *errorNorm)
 {
+  const CaseDataType eff = caseDataEffective(cdIn);
+  const CaseDataType *cd = &eff;
   const int nX = ld->Dims[0], nY = ld->Dims[1], nZ = ld->Dims[2];
   const PdfType visc = (F(1.0) / cd->Omega - F(0.5)) / F(3.0);
   PdfType *prof      = (PdfType *)calloc((size_t)nZ * nY, sizeof(PdfType));

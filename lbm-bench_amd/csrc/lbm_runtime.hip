@@ -389,8 +389,10 @@ static AuxLaunch auxOf(const LbmDevice *D, int rule)
   return A;
 }
 
-static void runKernel(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd)
+static void runKernel(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cdIn)
 {
+  const CaseDataType eff = caseDataEffective(cdIn);
+  const CaseDataType *cd = &eff;
   LbmDevice *D = kd->Dev;
   flushMirror(D);
   int n = cd->MaxIterations;
@@ -489,10 +491,12 @@ static void checkGeometry(const LatticeDescType *ld)
       }
 }
 
-static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataType *cd,
+static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataType *cdIn,
     int prop, int layout, int tiled, const char *name,
     void (*kernel)(LatticeDescType *, KernelDataType *, CaseDataType *))
 {
+  const CaseDataType eff = caseDataEffective(cdIn);
+  const CaseDataType *cd = &eff;
   checkGeometry(ld);
   KernelDataType *kd = (KernelDataType *)calloc(1, sizeof(KernelDataType));
   LbmDevice *D       = new LbmDevice();
@@ -651,8 +655,10 @@ void kernelInitAaSoA(LatticeDescType *ld, KernelDataType **kd, CaseDataType *cd)
   initInternal(ld, kd, cd, PROP_AA, LAYOUT_SOA, 0, "aa-soa", kernelAaSoA);
 }
 
-void kernelAddBodyForce(KernelDataType *kd, LatticeDescType *ld, CaseDataType *cd)
+void kernelAddBodyForce(KernelDataType *kd, LatticeDescType *ld, CaseDataType *cdIn)
 {
+  const CaseDataType eff = caseDataEffective(cdIn);
+  const CaseDataType *cd = &eff;
   LbmDevice *D = kd->Dev;
   flushMirror(D);
   const StepLaunch base = baseLaunch(D, cd);

@@ -43,7 +43,7 @@ def _structs(pdf):
     class CaseData(ctypes.Structure):
         _fields_ = [("Omega", pdf), ("RhoIn", pdf), ("RhoOut", pdf), ("XForce", pdf),
                     ("MaxIterations", c_int), ("StatisticsModulus", c_int),
-                    ("MagicParam", c_double), ("BodyForce", c_int), ("Storage", c_int),
+                    ("Extensions", c_int), ("MagicParam", c_double), ("BodyForce", c_int), ("Storage", c_int),
                     ("StorageScale", c_double), ("RhoRef", c_double)]
 
     class KernelData(ctypes.Structure):

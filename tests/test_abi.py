@@ -139,3 +139,16 @@ def test_fluid_on_open_face_is_rejected():
             % os.path.dirname(lbmhip.__file__))
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert r.returncode != 0 and "non-periodic boundary" in r.stderr
+
+
+def test_reference_driver_links_the_gpu_library():
+    """LBM-Bench's unmodified main.c/cli.c built against include/lbm.h"""
+    exe = os.path.join(ROOT, "oracle", "_ref", "lbmbench-ref-driver-hip-dp")
+    if not os.path.exists(exe):
+        pytest.skip("reference driver not built")
+    out = subprocess.check_output(["ldd", exe], text=True)
+    assert "liblbmhip-dp.so" in out
+    r = subprocess.run([exe, "-l"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0
+    listed = [l.strip() for l in r.stdout.split("Available kernels:")[1].splitlines() if l.strip()]
+    assert listed == list(lbmhip.KERNELS)

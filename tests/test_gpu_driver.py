@@ -45,3 +45,29 @@ def test_benchmark_output_format():
     assert abs(drift) < 1e-13
     for line in ("# Evaluation Stats", "#   runtime:", "#   MEM bandwidth:", "#   performance:"):
         assert line in r.stdout
+
+
+REF_DRIVER = os.path.join(os.path.dirname(GOLDEN), "..", "oracle", "_ref", "lbmbench-ref-driver-hip-{}")
+
+
+@pytest.mark.parametrize("precision", ["dp", "sp"])
+def test_reference_driver_runs_on_the_gpu(precision):
+    """LBM-Bench's own main.c and cli.c, unmodified, compiled against
+    include/lbm.h and linked to the GPU library: its -V run (the body force
+    of its -DVERIFICATION build switched on by LBM_BODY_FORCE) reproduces
+    the norm the reference prints"""
+    exe = os.path.abspath(REF_DRIVER.format(precision))
+    if not os.path.exists(exe):
+        pytest.skip("reference driver not built (needs LBM-Bench's tree at build time)")
+    with open(os.path.join(GOLDEN, "verification.json")) as f:
+        ver = json.load(f)[precision]
+    spread = max(ver.values()) - min(ver.values())
+    env = dict(os.environ, LBM_BODY_FORCE="1")
+    for kernel in lbmhip.KERNELS:
+        r = subprocess.run([exe, "-V", "-k", kernel], capture_output=True, text=True, env=env,
+                           timeout=300)
+        assert r.returncode == 0, r.stdout[-2000:]
+        assert "on the GPU" in r.stdout
+        norm = float(re.search(r"deviation from analytical solution: (\S+)", r.stdout).group(1))
+        tol = 1e-6 * ver[kernel] if precision == "dp" else 2 * spread
+        assert abs(norm - ver[kernel]) <= tol, (kernel, norm, ver[kernel])
