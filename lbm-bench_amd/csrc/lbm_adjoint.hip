@@ -40,7 +40,7 @@ struct AdjParams {
   R scale, inv;  // forward storage scale
   double rhoRef; // forward deviation offset (0 for FP64 storage)
   uint64_t plane;
-  int nx, ny, nz, segs, outletX, remap;
+  int nx, ny, nz, segs, outletX, remap, dither;
 };
 
 // TRT collision with the Brinkman-scaled momentum (keep = 1 - s); at
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void kBrinkmanStep(const SF *src, SF *dst,
       i = (uint64_t)d * p.plane + c.cellOf(cx(d), cy(d), cz(d));
       v = f[d] + forceOf<R>(d, p.f1, p.f2);
     }
-    putS<true>(&dst[i], St::store(v, p.scale));
+    putS<true>(&dst[i], St::storeStep(v, p.scale, (uint32_t)i, p.dither));
   }
 }
 
@@ -286,7 +286,7 @@ struct LbmAdjoint {
   uint64_t ncells;
   double rhoRef;
   float scale, inv;
-  int outletX, remap;
+  int outletX, remap, dither;
   CaseDataType cd;
   void *sfield;        // R per cell
   double *grad;
@@ -348,6 +348,7 @@ static AdjParams<R> adjParams(const LbmAdjoint *A)
   p.segs   = A->g.segs;
   p.outletX = A->outletX;
   p.remap  = A->remap;
+  p.dither = A->dither;
   return p;
 }
 
@@ -575,6 +576,8 @@ LbmAdjoint *lbmAdjointCreate(LatticeDescType *ld, CaseDataType *cd, int outletX)
   A->outletX    = outletX;
   const char *rm = getenv("LBM_XCD_REMAP");
   A->remap      = rm && *rm ? atoi(rm) : 1;
+  const char *di = getenv("LBM_DITHER");
+  A->dither     = di && *di ? atoi(di) : 1;
   switch (cd->Storage) {
   case LBM_STORE_NATIVE:
     A->sf = ST_F64;

@@ -100,6 +100,25 @@ __device__ __forceinline__ void collide(R (&f)[Q], const Trt<R> &p)
 // the FP32 compute type is one exact multiply.  The collision is applied
 // to the deviation directly: the incompressible equilibrium is linear in
 // rho, so the w*RhoRef part drops out of every term.
+//
+// store() rounds to nearest even.  storeStep() is what the step kernels
+// call: for FP16 and BF16 it can round by dithering instead, adding a
+// hash of (value bits, slot index) to the bits that are dropped and then
+// truncating.  Each rounding is then unbiased, so a slowly varying field
+// (and a small body force added to it) does not collect the same rounding
+// error step after step.
+
+// 32-bit integer hash (lowbias32: Wellons 2018)
+__device__ __forceinline__ uint32_t mix32(uint32_t x)
+{
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
 template <typename S>
 struct Store;
 template <>
@@ -108,6 +127,7 @@ struct Store<double> {
   static constexpr bool scaled = false;
   __device__ static R load(double v, R) { return v; }
   __device__ static double store(R v, R) { return v; }
+  __device__ static double storeStep(R v, R sc, uint32_t, int) { return v; }
 };
 template <>
 struct Store<float> {
@@ -115,6 +135,7 @@ struct Store<float> {
   static constexpr bool scaled = false;
   __device__ static R load(float v, R) { return v; }
   __device__ static float store(R v, R) { return v; }
+  __device__ static float storeStep(R v, R, uint32_t, int) { return v; }
 };
 // FP32 shifted-deviation storage: same bytes as float, scaled
 struct f32dev {
@@ -126,6 +147,7 @@ struct Store<f32dev> {
   static constexpr bool scaled = true;
   __device__ static R load(f32dev v, R inv) { return v.v * inv; }
   __device__ static f32dev store(R v, R sc) { return f32dev{ v * sc }; }
+  __device__ static f32dev storeStep(R v, R sc, uint32_t, int) { return f32dev{ v * sc }; }
 };
 template <>
 struct Store<__half> {
@@ -133,6 +155,20 @@ struct Store<__half> {
   static constexpr bool scaled = true;
   __device__ static R load(__half v, R inv) { return __half2float(v) * inv; }
   __device__ static __half store(R v, R sc) { return __float2half_rn(v * sc); }
+  __device__ static __half storeStep(R v, R sc, uint32_t salt, int dither)
+  {
+    const float x = v * sc;
+    if (dither) {
+      uint32_t u       = __float_as_uint(x);
+      const uint32_t a = u & 0x7fffffffu;
+      // normal FP16 range [2^-14, 65504]: 13 float mantissa bits are dropped
+      if (a >= 0x38800000u && a < 0x477fe000u) {
+        u = (u + (mix32(u ^ salt) & 0x1fffu)) & ~0x1fffu;
+        return __float2half_rz(__uint_as_float(u));
+      }
+    }
+    return __float2half_rn(x);
+  }
 };
 template <>
 struct Store<__hip_bfloat16> {
@@ -140,6 +176,20 @@ struct Store<__hip_bfloat16> {
   static constexpr bool scaled = true;
   __device__ static R load(__hip_bfloat16 v, R inv) { return __bfloat162float(v) * inv; }
   __device__ static __hip_bfloat16 store(R v, R sc) { return __float2bfloat16(v * sc); }
+  __device__ static __hip_bfloat16 storeStep(R v, R sc, uint32_t salt, int dither)
+  {
+    const float x = v * sc;
+    if (dither) {
+      uint32_t u       = __float_as_uint(x);
+      const uint32_t a = u & 0x7fffffffu;
+      // finite and normal: the low 16 bits are dropped
+      if (a >= 0x00800000u && a < 0x7f7f0000u) {
+        u = (u + (mix32(u ^ salt) & 0xffffu)) & 0xffff0000u;
+        return __float2bfloat16(__uint_as_float(u)); // exact after truncation
+      }
+    }
+    return __float2bfloat16(x);
+  }
 };
 
 // ---- solid mask -----------------------------------------------------------

@@ -16,6 +16,7 @@ struct Params {
   uint64_t plane;
   int nx, ny, nz, segs, nrows;
   int addForce;
+  int dither; // FP16/BF16: round by dithering instead of to nearest
 };
 
 template <typename R>
@@ -48,6 +49,7 @@ inline Params<S> makeParams(const StepLaunch &L)
   p.segs     = L.g.segs;
   p.nrows    = L.g.nx * L.g.ny;
   p.addForce = L.addForce;
+  p.dither   = L.dither;
   return p;
 }
 

@@ -42,6 +42,7 @@ struct StepLaunch {
   // launch shape: XCD-aware block order, non-temporal stores (bit 0) /
   // loads (bit 1), wavefronts per workgroup (4 or 8)
   int remap, nt, wavesPerBlock;
+  int dither; // FP16/BF16 storage: dithered rounding in the step
 };
 
 void launchStepRow(const StepLaunch &L, hipStream_t s);  // lbm_step.hip

@@ -78,7 +78,7 @@ struct LbmDevice {
   float scale, invScale;
   double *dpart, *dout;
   int npart;
-  int remap, nt, wavesPerBlock;
+  int remap, nt, wavesPerBlock, dither;
 };
 
 // ---- small helpers ---------------------------------------------------------
@@ -310,6 +310,7 @@ static StepLaunch baseLaunch(const LbmDevice *D, const CaseDataType *cd)
   L.remap         = D->remap;
   L.nt            = D->nt;
   L.wavesPerBlock = D->wavesPerBlock;
+  L.dither        = D->dither;
   return L;
 }
 
@@ -553,6 +554,7 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
   D->remap   = envInt("LBM_XCD_REMAP", 1);
   D->nt      = envInt("LBM_NT", 3);
   D->wavesPerBlock = envInt("LBM_WAVES_PER_BLOCK", 4);
+  D->dither        = envInt("LBM_DITHER", 1);
   kd->LoopBalance = 2.0 * Q * (double)D->esize;
 
   HIP_OK(hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking));

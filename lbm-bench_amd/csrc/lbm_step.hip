@@ -72,12 +72,16 @@ __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const
 
   if constexpr (V == V_PULL) {
 #pragma unroll
-    for (int d = 0; d < Q; ++d)
-      putS<NTS>(&b[slot<AOS>(cell, d, p.plane)], St::store(f[d], p.scale));
+    for (int d = 0; d < Q; ++d) {
+      const uint64_t i = slot<AOS>(cell, d, p.plane);
+      putS<NTS>(&b[i], St::storeStep(f[d], p.scale, (uint32_t)i, p.dither));
+    }
   } else if constexpr (V == V_AA_EVEN) {
 #pragma unroll
-    for (int d = 0; d < Q; ++d)
-      putS<NTS>(&a[slot<AOS>(cell, inv(d), p.plane)], St::store(f[d], p.scale));
+    for (int d = 0; d < Q; ++d) {
+      const uint64_t i = slot<AOS>(cell, inv(d), p.plane);
+      putS<NTS>(&a[i], St::storeStep(f[d], p.scale, (uint32_t)i, p.dither));
+    }
   } else {
     S *o = V == V_PUSH ? b : a;
 #pragma unroll
@@ -89,7 +93,7 @@ __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const
         const bool bb = solidOf(cx(d), cy(d), cz(d));
         i = bb ? slot<AOS>(cell, inv(d), p.plane) : slot<AOS>(cellOf(cx(d), cy(d), cz(d)), d, p.plane);
       }
-      putS<NTS>(&o[i], St::store(f[d], p.scale));
+      putS<NTS>(&o[i], St::storeStep(f[d], p.scale, (uint32_t)i, p.dither));
     }
   }
 }

@@ -57,8 +57,10 @@ __global__ __launch_bounds__(512) void kStepTile(S *a, S *b, const MaskWin *__re
   collide(f, p.trt);
   if constexpr (V == V_PULL) {
 #pragma unroll
-    for (int d = 0; d < Q; ++d)
-      b[slot<false>(cell, d, p.plane)] = St::store(f[d], p.scale);
+    for (int d = 0; d < Q; ++d) {
+      const uint64_t i = slot<false>(cell, d, p.plane);
+      b[i] = St::storeStep(f[d], p.scale, (uint32_t)i, p.dither);
+    }
   } else {
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
@@ -68,7 +70,7 @@ __global__ __launch_bounds__(512) void kStepTile(S *a, S *b, const MaskWin *__re
         const uint64_t n = nbr(cx(d), cy(d), cz(d), bb);
         i = bb ? slot<false>(cell, inv(d), p.plane) : slot<false>(n, d, p.plane);
       }
-      b[i] = St::store(f[d], p.scale);
+      b[i] = St::storeStep(f[d], p.scale, (uint32_t)i, p.dither);
     }
   }
 }

@@ -74,9 +74,10 @@ def test_one_step_within_format_roundoff(gpu_lib, oracle, fmt, kernel):
     dev = np.abs(_dev(exp)[fluid]).max()
     err = np.abs(got - exp)[fluid].max()
     # two steps and the closing force pass each round the stored deviation
-    # once (<= u |dev|); the FP32 collision works on the deviation itself, so
-    # its own error is a few 2^-24 |dev|
-    assert err <= (4 * U[fmt] + 64 * 2.0 ** -24) * dev, (err, dev)
+    # once (<= u |dev| to nearest, < 2u |dev| by dithering); the FP32
+    # collision works on the deviation itself, so its own error is a few
+    # 2^-24 |dev|
+    assert err <= (8 * U[fmt] + 64 * 2.0 ** -24) * dev, (err, dev)
 
 
 @pytest.fixture(scope="module")
