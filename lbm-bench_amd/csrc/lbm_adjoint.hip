@@ -41,6 +41,7 @@ struct AdjParams {
   double rhoRef; // forward deviation offset (0 for FP64 storage)
   uint64_t plane;
   int nx, ny, nz, segs, outletX, remap, dither;
+  uint32_t seq; // forward step index, for the rounding seeds
 };
 
 // TRT collision with the Brinkman-scaled momentum (keep = 1 - s); at
@@ -106,6 +107,7 @@ __global__ __launch_bounds__(256) void kBrinkmanStep(const SF *src, SF *dst,
   for (int d = 0; d < Q; ++d)
     f[d] = St::load(getS<true>(&src[(uint64_t)d * p.plane + c.cell]), p.inv);
   collideBrinkman(f, p.trt, R(1) - sfield[c.cell]);
+  const Dither dz = ditherOf(c.cell, p.seq, p.dither);
 #pragma unroll
   for (int d = 0; d < Q; ++d) {
     uint64_t i;
@@ -120,7 +122,7 @@ __global__ __launch_bounds__(256) void kBrinkmanStep(const SF *src, SF *dst,
       i = (uint64_t)d * p.plane + c.cellOf(cx(d), cy(d), cz(d));
       v = f[d] + forceOf<R>(d, p.f1, p.f2);
     }
-    putS<true>(&dst[i], St::storeStep(v, p.scale, (uint32_t)i, p.dither));
+    putS<true>(&dst[i], St::storeStep(v, p.scale, ditherDir(dz, d)));
   }
 }
 
@@ -349,6 +351,7 @@ static AdjParams<R> adjParams(const LbmAdjoint *A)
   p.outletX = A->outletX;
   p.remap  = A->remap;
   p.dither = A->dither;
+  p.seq    = 0;
   return p;
 }
 
@@ -362,22 +365,34 @@ static void *adjAlloc(LbmAdjoint *A, size_t bytes)
   return p;
 }
 
-static void adjForwardStep(LbmAdjoint *A, const void *src, void *dst)
+// forward step t -> t + 1.  The FP16 rounding seeds depend on t, so a state
+// recomputed from a checkpoint equals the one of the first sweep bit for bit.
+static void adjForwardStep(LbmAdjoint *A, const void *src, void *dst, int t)
 {
   const unsigned blocks = rowBlocks(A->g, 4);
+  const uint32_t seq    = (uint32_t)t * 0x9e3779b9u;
   switch (A->sf) {
-  case ST_F64:
+  case ST_F64: {
+    AdjParams<double> p = adjParams<double>(A);
+    p.seq               = seq;
     hipLaunchKernelGGL(kBrinkmanStep<double>, dim3(blocks), dim3(256), 0, A->stream,
-        (const double *)src, (double *)dst, (const double *)A->sfield, A->g.mask, adjParams<double>(A));
+        (const double *)src, (double *)dst, (const double *)A->sfield, A->g.mask, p);
     break;
-  case ST_F32DEV:
+  }
+  case ST_F32DEV: {
+    AdjParams<float> p = adjParams<float>(A);
+    p.seq              = seq;
     hipLaunchKernelGGL(kBrinkmanStep<f32dev>, dim3(blocks), dim3(256), 0, A->stream,
-        (const f32dev *)src, (f32dev *)dst, (const float *)A->sfield, A->g.mask, adjParams<float>(A));
+        (const f32dev *)src, (f32dev *)dst, (const float *)A->sfield, A->g.mask, p);
     break;
-  default:
+  }
+  default: {
+    AdjParams<float> p = adjParams<float>(A);
+    p.seq              = seq;
     hipLaunchKernelGGL(kBrinkmanStep<__half>, dim3(blocks), dim3(256), 0, A->stream,
-        (const __half *)src, (__half *)dst, (const float *)A->sfield, A->g.mask, adjParams<float>(A));
+        (const __half *)src, (__half *)dst, (const float *)A->sfield, A->g.mask, p);
     break;
+  }
   }
   HIP_OK(hipGetLastError());
 }
@@ -454,10 +469,10 @@ static float eventSeconds(LbmAdjoint *A)
   return ms * 1e-3f;
 }
 
-// run n forward steps from src; the last state lands in dst (src is kept).
-// Outputs alternate between dst and a scratch buffer so that the last one
-// is dst: step i writes dst when n - 1 - i is even.
-static void adjForwardRun(LbmAdjoint *A, const void *src, void *dst, int n)
+// run n forward steps from src = f^{t0}; the last state lands in dst (src
+// is kept).  Outputs alternate between dst and a scratch buffer so that the
+// last one is dst: step i writes dst when n - 1 - i is even.
+static void adjForwardRun(LbmAdjoint *A, const void *src, void *dst, int n, int t0)
 {
   if (n <= 0) {
     HIP_OK(hipMemcpyAsync(dst, src, stateBytes(A), hipMemcpyDeviceToDevice, A->stream));
@@ -471,7 +486,7 @@ static void adjForwardRun(LbmAdjoint *A, const void *src, void *dst, int n)
   const void *cur = src;
   for (int i = 0; i < n; ++i) {
     void *out = ((n - 1 - i) % 2 == 0) ? dst : scratch;
-    adjForwardStep(A, cur, out);
+    adjForwardStep(A, cur, out, t0 + i);
     cur = out;
   }
 }
@@ -486,7 +501,7 @@ static std::vector<void *> adjCheckpoints(LbmAdjoint *A, int K, int t0, int t1, 
   ck[0] = start;
   for (int i = 1; i < m; ++i) {
     ck[i] = A->pool[base + i - 1];
-    adjForwardRun(A, ck[i - 1], ck[i], K);
+    adjForwardRun(A, ck[i - 1], ck[i], K, t0 + (i - 1) * K);
   }
   return ck;
 }
@@ -519,7 +534,7 @@ static void adjBackwardRange(LbmAdjoint *A, const int *intervals, int nlevels, i
     HIP_OK(hipEventRecord(A->ev0, A->stream));
     for (int i = 1; i < n; ++i) {
       st[i] = A->pool[base + i - 1];
-      adjForwardStep(A, st[i - 1], st[i]);
+      adjForwardStep(A, st[i - 1], st[i], t0 + i - 1);
     }
     HIP_OK(hipEventRecord(A->ev1, A->stream));
     A->tRecompute += eventSeconds(A);
@@ -702,7 +717,7 @@ void lbmAdjointInitState(LbmAdjoint *A, int mode, uint64_t seed, double rho, dou
 double lbmAdjointForward(LbmAdjoint *A, int steps, double *seconds)
 {
   HIP_OK(hipEventRecord(A->ev0, A->stream));
-  adjForwardRun(A, A->state0, A->work[0], steps);
+  adjForwardRun(A, A->state0, A->work[0], steps, 0);
   HIP_OK(hipEventRecord(A->ev1, A->stream));
   const double t = eventSeconds(A);
   if (seconds)
@@ -746,9 +761,9 @@ double lbmAdjointGradient(LbmAdjoint *A, int steps, const int *intervals, int nl
     st[0] = A->state0;
     for (int i = 1; i < steps; ++i) {
       st[i] = A->pool[i - 1];
-      adjForwardStep(A, st[i - 1], st[i]);
+      adjForwardStep(A, st[i - 1], st[i], i - 1);
     }
-    adjForwardStep(A, st[steps - 1], A->work[0]);
+    adjForwardStep(A, st[steps - 1], A->work[0], steps - 1);
     HIP_OK(hipEventRecord(A->ev1, A->stream));
     A->tForward = eventSeconds(A);
     J           = adjObjective(A, A->work[0], A->lam[0]);
@@ -764,7 +779,7 @@ double lbmAdjointGradient(LbmAdjoint *A, int steps, const int *intervals, int nl
     HIP_OK(hipEventRecord(A->ev0, A->stream));
     const std::vector<void *> ck = adjCheckpoints(A, intervals[0], 0, steps, A->state0, 0);
     const int last = (int)(ck.size() - 1) * intervals[0];
-    adjForwardRun(A, ck.back(), A->work[0], steps - last);
+    adjForwardRun(A, ck.back(), A->work[0], steps - last, last);
     HIP_OK(hipEventRecord(A->ev1, A->stream));
     A->tForward = eventSeconds(A);
     J           = adjObjective(A, A->work[0], A->lam[0]);
@@ -786,7 +801,7 @@ void lbmAdjointStepTimes(LbmAdjoint *A, int reps, double out[2])
 {
   HIP_OK(hipEventRecord(A->ev0, A->stream));
   for (int r = 0; r < reps; ++r)
-    adjForwardStep(A, A->work[r & 1], A->work[(r + 1) & 1]);
+    adjForwardStep(A, A->work[r & 1], A->work[(r + 1) & 1], r);
   HIP_OK(hipEventRecord(A->ev1, A->stream));
   out[0] = eventSeconds(A) / reps;
   HIP_OK(hipEventRecord(A->ev0, A->stream));

@@ -55,13 +55,14 @@ __device__ __forceinline__ CellPos cellPos(const Geom &g)
 // ---- body force pass -------------------------------------------------------
 template <typename S>
 __global__ __launch_bounds__(256) void kForce(S *a, Geom g, int rule, float scale, float inv,
-    double f1, double f2)
+    double f1, double f2, int dither, uint32_t seq)
 {
   using R  = typename Store<S>::R;
   using St = Store<S>;
   const CellPos c = cellPos(g);
   if (!c.valid)
     return;
+  const Dither dz = ditherOf(c.cell, seq * 0x9e3779b9u, dither);
 #pragma unroll
   for (int d = 0; d < Q; ++d) {
     if (cx(d) == 0)
@@ -70,8 +71,19 @@ __global__ __launch_bounds__(256) void kForce(S *a, Geom g, int rule, float scal
     const uint64_t i = logicalSlot(g, rule, c.x, c.y, c.z, c.cell, d);
     R v              = St::load(a[i], (R)inv);
     v += cx(d) > 0 ? fd : -fd;
-    a[i] = St::store(v, (R)scale);
+    a[i] = St::storeStep(v, (R)scale, ditherDir(dz, d));
   }
+}
+
+__global__ void kBumpSeq(uint32_t *counter, uint32_t n)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+    counter[0] += n;
+}
+
+void launchBumpSeq(uint32_t *counter, uint32_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL(kBumpSeq, dim3(1), dim3(64), 0, s, counter, n);
 }
 
 void launchForce(const AuxLaunch &A, double f1, double f2, hipStream_t s)
@@ -80,7 +92,7 @@ void launchForce(const AuxLaunch &A, double f1, double f2, hipStream_t s)
   const unsigned blocks = (unsigned)((n + 255) / 256);
 #define LAUNCH(T)                                                                          \
   hipLaunchKernelGGL(kForce<T>, dim3(blocks), dim3(256), 0, s, (T *)A.pdf, A.g, A.rule, A.scale, \
-      A.invScale, f1, f2)
+      A.invScale, f1, f2, A.dither, A.seq)
   switch (A.storage) {
   case ST_F64: LAUNCH(double); break;
   case ST_F32: LAUNCH(float); break;

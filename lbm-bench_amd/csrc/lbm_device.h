@@ -102,11 +102,18 @@ __device__ __forceinline__ void collide(R (&f)[Q], const Trt<R> &p)
 // rho, so the w*RhoRef part drops out of every term.
 //
 // store() rounds to nearest even.  storeStep() is what the step kernels
-// call: for FP16 and BF16 it can round by dithering instead, adding a
-// hash of (value bits, slot index) to the bits that are dropped and then
-// truncating.  Each rounding is then unbiased, so a slowly varying field
-// (and a small body force added to it) does not collect the same rounding
-// error step after step.
+// call: for FP16 and BF16 it rounds with gfx950's stochastic-rounding
+// converts (v_cvt_sr_f16_f32, v_cvt_sr_bf16_f32).  These add bits 16..31
+// of a 32-bit seed below the last kept bit and truncate, so a seed that is
+// uniform over those bits rounds up with probability (dropped part /
+// quantum): each rounding is unbiased, and a slowly varying field (or a
+// small body force added to it) does not collect the same rounding error
+// step after step.  The seed is (value bits & mask) ^ salt.  FP16 drops 13
+// and BF16 16 bits, so bits 16..31 of the value are kept bits only and the
+// seed never depends on the part being rounded.  The salt hashes the cell
+// index with a per-step sequence number and differs per direction.  With
+// dithering off, mask is 0 and salt 0x80000000: half a quantum is added
+// and the result truncated, which is round to nearest, ties away from 0.
 
 // 32-bit integer hash (lowbias32: Wellons 2018)
 __device__ __forceinline__ uint32_t mix32(uint32_t x)
@@ -119,6 +126,24 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x)
   return x;
 }
 
+struct Dither {
+  uint32_t mask, salt;
+};
+// per cell: on != 0 dithers, seq varies from step to step
+__device__ __forceinline__ Dither ditherOf(uint64_t cell, uint32_t seq, int on)
+{
+  const uint32_t h = mix32((uint32_t)cell ^ (uint32_t)(cell >> 32) ^ seq);
+  return on ? Dither{ 0xffffffffu, h } : Dither{ 0u, 0x80000000u };
+}
+// per direction: the cell's salt xor'ed with a direction constant
+__device__ __forceinline__ Dither ditherDir(Dither z, int d)
+{
+  return Dither{ z.mask, z.salt ^ (((uint32_t)d + 1u) * 0x9e3779b9u & z.mask) };
+}
+
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf162v __attribute__((ext_vector_type(2)));
+
 template <typename S>
 struct Store;
 template <>
@@ -127,7 +152,7 @@ struct Store<double> {
   static constexpr bool scaled = false;
   __device__ static R load(double v, R) { return v; }
   __device__ static double store(R v, R) { return v; }
-  __device__ static double storeStep(R v, R sc, uint32_t, int) { return v; }
+  __device__ static double storeStep(R v, R, Dither) { return v; }
 };
 template <>
 struct Store<float> {
@@ -135,7 +160,7 @@ struct Store<float> {
   static constexpr bool scaled = false;
   __device__ static R load(float v, R) { return v; }
   __device__ static float store(R v, R) { return v; }
-  __device__ static float storeStep(R v, R, uint32_t, int) { return v; }
+  __device__ static float storeStep(R v, R, Dither) { return v; }
 };
 // FP32 shifted-deviation storage: same bytes as float, scaled
 struct f32dev {
@@ -147,7 +172,7 @@ struct Store<f32dev> {
   static constexpr bool scaled = true;
   __device__ static R load(f32dev v, R inv) { return v.v * inv; }
   __device__ static f32dev store(R v, R sc) { return f32dev{ v * sc }; }
-  __device__ static f32dev storeStep(R v, R sc, uint32_t, int) { return f32dev{ v * sc }; }
+  __device__ static f32dev storeStep(R v, R sc, Dither) { return f32dev{ v * sc }; }
 };
 template <>
 struct Store<__half> {
@@ -155,19 +180,12 @@ struct Store<__half> {
   static constexpr bool scaled = true;
   __device__ static R load(__half v, R inv) { return __half2float(v) * inv; }
   __device__ static __half store(R v, R sc) { return __float2half_rn(v * sc); }
-  __device__ static __half storeStep(R v, R sc, uint32_t salt, int dither)
+  __device__ static __half storeStep(R v, R sc, Dither z)
   {
-    const float x = v * sc;
-    if (dither) {
-      uint32_t u       = __float_as_uint(x);
-      const uint32_t a = u & 0x7fffffffu;
-      // normal FP16 range [2^-14, 65504]: 13 float mantissa bits are dropped
-      if (a >= 0x38800000u && a < 0x477fe000u) {
-        u = (u + (mix32(u ^ salt) & 0x1fffu)) & ~0x1fffu;
-        return __float2half_rz(__uint_as_float(u));
-      }
-    }
-    return __float2half_rn(x);
+    const float x       = v * sc;
+    const uint32_t seed = (__float_as_uint(x) & z.mask) ^ z.salt;
+    const half2v r      = __builtin_amdgcn_cvt_sr_f16_f32(half2v{ 0, 0 }, x, seed, false);
+    return __half(__half_raw{ r[0] });
   }
 };
 template <>
@@ -176,19 +194,12 @@ struct Store<__hip_bfloat16> {
   static constexpr bool scaled = true;
   __device__ static R load(__hip_bfloat16 v, R inv) { return __bfloat162float(v) * inv; }
   __device__ static __hip_bfloat16 store(R v, R sc) { return __float2bfloat16(v * sc); }
-  __device__ static __hip_bfloat16 storeStep(R v, R sc, uint32_t salt, int dither)
+  __device__ static __hip_bfloat16 storeStep(R v, R sc, Dither z)
   {
-    const float x = v * sc;
-    if (dither) {
-      uint32_t u       = __float_as_uint(x);
-      const uint32_t a = u & 0x7fffffffu;
-      // finite and normal: the low 16 bits are dropped
-      if (a >= 0x00800000u && a < 0x7f7f0000u) {
-        u = (u + (mix32(u ^ salt) & 0xffffu)) & 0xffff0000u;
-        return __float2bfloat16(__uint_as_float(u)); // exact after truncation
-      }
-    }
-    return __float2bfloat16(x);
+    const float x       = v * sc;
+    const uint32_t seed = (__float_as_uint(x) & z.mask) ^ z.salt;
+    const bf162v r      = __builtin_amdgcn_cvt_sr_bf16_f32(bf162v{ 0, 0 }, x, seed, false);
+    return __hip_bfloat16(__hip_bfloat16_raw{ __builtin_bit_cast(unsigned short, r[0]) });
   }
 };
 

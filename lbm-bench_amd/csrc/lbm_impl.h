@@ -16,7 +16,9 @@ struct Params {
   uint64_t plane;
   int nx, ny, nz, segs, nrows;
   int addForce;
-  int dither; // FP16/BF16: round by dithering instead of to nearest
+  int dither;   // FP16/BF16: stochastic rounding instead of to nearest
+  const uint32_t *seqBase; // step number of the rounding seeds:
+  uint32_t seq;            // *seqBase + seq
 };
 
 template <typename R>
@@ -50,7 +52,16 @@ inline Params<S> makeParams(const StepLaunch &L)
   p.nrows    = L.g.nx * L.g.ny;
   p.addForce = L.addForce;
   p.dither   = L.dither;
+  p.seqBase  = L.seqBase;
+  p.seq      = L.seq;
   return p;
+}
+
+// seed input of a step: its step number, spread over the 32 bits
+template <typename S>
+__device__ __forceinline__ uint32_t stepSeq(const Params<S> &p)
+{
+  return (*p.seqBase + p.seq) * 0x9e3779b9u;
 }
 
 // element index of (cell, d)

@@ -42,7 +42,11 @@ struct StepLaunch {
   // launch shape: XCD-aware block order, non-temporal stores (bit 0) /
   // loads (bit 1), wavefronts per workgroup (4 or 8)
   int remap, nt, wavesPerBlock;
-  int dither; // FP16/BF16 storage: dithered rounding in the step
+  int dither;    // FP16/BF16 storage: stochastic rounding in the step
+  // rounding seeds: step number = *seqBase + seq (seqBase in device memory,
+  // advanced by every replay of a step graph; see launchBumpSeq)
+  const uint32_t *seqBase;
+  uint32_t seq;
 };
 
 void launchStepRow(const StepLaunch &L, hipStream_t s);  // lbm_step.hip
@@ -55,9 +59,14 @@ struct AuxLaunch {
   void *pdf;
   double rhoRef;       // shifted-deviation reference density
   float scale, invScale;
+  int dither;          // kForce: stochastic rounding of FP16/BF16, seeded by seq
+  uint32_t seq;
 };
 // f_d += 3 w_d c_dx g at the slot holding logical f_d (src/lbm.c:902-922)
 void launchForce(const AuxLaunch &A, double f1, double f2, hipStream_t s);
+// *counter += n, one lane: the last node of a step graph, so that each
+// replay seeds its rounding with new step numbers
+void launchBumpSeq(uint32_t *counter, uint32_t n, hipStream_t s);
 // fill logical PDFs: mode 0 zero, 1 w_d*rho, 2 w_d*(rho + (2u-1)*amp)
 void launchInit(const AuxLaunch &A, int mode, uint64_t seed, double rho, double amp,
     hipStream_t s);

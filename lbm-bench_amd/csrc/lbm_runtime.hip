@@ -6,8 +6,10 @@
 // Device state per KernelData (struct LbmDevice):
 //   one (AA) or two (push/pull) PDF arrays, hipMalloc'd, padded SoA planes
 //   or AoS; the solid-mask windows; a non-blocking stream of default
-//   priority; two hipEvents; hipGraphs of G consecutive steps; a host mirror
-//   of the array holding the state, allocated on the first GetNode/SetNode.
+//   priority; two hipEvents; hipGraphs of G consecutive steps; a device
+//   step counter that seeds the FP16/BF16 stochastic rounding; a host
+//   mirror of the array holding the state, allocated on the first
+//   GetNode/SetNode.
 //
 // Step loop of kernel<Variant> (the time loop of src/kernel_*.c):
 //   the first step of a run collides the state as it is; every later step
@@ -79,6 +81,12 @@ struct LbmDevice {
   double *dpart, *dout;
   int npart;
   int remap, nt, wavesPerBlock, dither;
+  // FP16/BF16 rounding seeds.  dseq (device) counts the steps run by
+  // graphs: each node of a graph adds its index to it and the graph's last
+  // node advances it.  Eager steps and force passes number themselves
+  // from seq, offset by 2^31 so that the two ranges stay apart.
+  uint32_t *dseq;
+  uint32_t seq;
 };
 
 // ---- small helpers ---------------------------------------------------------
@@ -314,12 +322,17 @@ static StepLaunch baseLaunch(const LbmDevice *D, const CaseDataType *cd)
   return L;
 }
 
-// one step; for two-lattice variants src is D->d[srcIdx]
-static void launchStep(const LbmDevice *D, StepLaunch L, int variant, int srcIdx, int addForce,
-    hipStream_t s)
+static uint32_t eagerSeq(LbmDevice *D) { return 0x80000000u + D->seq++; }
+
+// one step; for two-lattice variants src is D->d[srcIdx].  node >= 0 is the
+// step's index inside a graph being captured, -1 an eager launch.
+static void launchStep(LbmDevice *D, StepLaunch L, int variant, int srcIdx, int addForce,
+    hipStream_t s, int node = -1)
 {
   L.variant  = variant;
   L.addForce = addForce;
+  L.seqBase  = D->dseq;
+  L.seq      = node >= 0 ? (uint32_t)node : eagerSeq(D);
   if (D->prop == PROP_AA) {
     L.src = L.dst = D->d[0];
   } else {
@@ -364,12 +377,14 @@ static void prepareGraphs(LbmDevice *D, const CaseDataType *cd, const StepLaunch
     int src = gi;
     for (int s = 0; s < D->gsteps; ++s) {
       if (D->prop == PROP_AA)
-        launchStep(D, base, (s & 1) ? V_AA_ODD : V_AA_EVEN, 0, cd->BodyForce, D->stream);
+        launchStep(D, base, (s & 1) ? V_AA_ODD : V_AA_EVEN, 0, cd->BodyForce, D->stream, s);
       else {
-        launchStep(D, base, stepVariant(D), src, cd->BodyForce, D->stream);
+        launchStep(D, base, stepVariant(D), src, cd->BodyForce, D->stream, s);
         src ^= 1;
       }
     }
+    launchBumpSeq(D->dseq, (uint32_t)D->gsteps, D->stream);
+    HIP_OK(hipGetLastError());
     HIP_OK(hipStreamEndCapture(D->stream, &graph));
     HIP_OK(hipGraphInstantiate(&D->gexec[gi], graph, nullptr, nullptr, 0));
     HIP_OK(hipGraphDestroy(graph));
@@ -387,6 +402,8 @@ static AuxLaunch auxOf(const LbmDevice *D, int rule)
   A.rhoRef   = D->rhoRef;
   A.scale    = D->scale;
   A.invScale = D->invScale;
+  A.dither   = D->dither;
+  A.seq      = 0;
   return A;
 }
 
@@ -438,7 +455,9 @@ static void runKernel(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cdI
     }
     if (bf) {
       const int rule = D->prop == PROP_PULL ? SR_PULL : SR_LOCAL;
-      launchForce(auxOf(D, rule), base.f1, base.f2, D->stream);
+      AuxLaunch A    = auxOf(D, rule);
+      A.seq          = eagerSeq(D);
+      launchForce(A, base.f1, base.f2, D->stream);
       HIP_OK(hipGetLastError());
     }
   }
@@ -555,6 +574,7 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
   D->nt      = envInt("LBM_NT", 3);
   D->wavesPerBlock = envInt("LBM_WAVES_PER_BLOCK", 4);
   D->dither        = envInt("LBM_DITHER", 1);
+  D->seq           = 0;
   kd->LoopBalance = 2.0 * Q * (double)D->esize;
 
   HIP_OK(hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking));
@@ -589,6 +609,8 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
   D->npart = momentsBlocks(geomOf(D));
   HIP_OK(hipMalloc(&D->dpart, sizeof(double) * 4 * (size_t)D->npart));
   HIP_OK(hipMalloc(&D->dout, sizeof(double) * 4));
+  HIP_OK(hipMalloc(&D->dseq, 256));
+  HIP_OK(hipMemset(D->dseq, 0, 256));
   pointMirror(kd);
 }
 
@@ -608,6 +630,7 @@ void kernelDeinit(LatticeDescType *ld, KernelDataType **kdp)
     HIP_OK(hipFree(D->dmask));
     HIP_OK(hipFree(D->dpart));
     HIP_OK(hipFree(D->dout));
+    HIP_OK(hipFree(D->dseq));
     HIP_OK(hipEventDestroy(D->ev0));
     HIP_OK(hipEventDestroy(D->ev1));
     HIP_OK(hipStreamDestroy(D->stream));
@@ -664,7 +687,9 @@ void kernelAddBodyForce(KernelDataType *kd, LatticeDescType *ld, CaseDataType *c
   LbmDevice *D = kd->Dev;
   flushMirror(D);
   const StepLaunch base = baseLaunch(D, cd);
-  launchForce(auxOf(D, currentRule(D, kd)), base.f1, base.f2, D->stream);
+  AuxLaunch A = auxOf(D, currentRule(D, kd));
+  A.seq       = eagerSeq(D);
+  launchForce(A, base.f1, base.f2, D->stream);
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(D->stream));
   D->mirrorValid = false;

@@ -69,18 +69,19 @@ __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const
     addForce(f, p.f1, p.f2);
 
   collide(f, p.trt);
+  const Dither dz = ditherOf(cell, stepSeq(p), p.dither);
 
   if constexpr (V == V_PULL) {
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
       const uint64_t i = slot<AOS>(cell, d, p.plane);
-      putS<NTS>(&b[i], St::storeStep(f[d], p.scale, (uint32_t)i, p.dither));
+      putS<NTS>(&b[i], St::storeStep(f[d], p.scale, ditherDir(dz, d)));
     }
   } else if constexpr (V == V_AA_EVEN) {
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
       const uint64_t i = slot<AOS>(cell, inv(d), p.plane);
-      putS<NTS>(&a[i], St::storeStep(f[d], p.scale, (uint32_t)i, p.dither));
+      putS<NTS>(&a[i], St::storeStep(f[d], p.scale, ditherDir(dz, d)));
     }
   } else {
     S *o = V == V_PUSH ? b : a;
@@ -93,7 +94,7 @@ __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const
         const bool bb = solidOf(cx(d), cy(d), cz(d));
         i = bb ? slot<AOS>(cell, inv(d), p.plane) : slot<AOS>(cellOf(cx(d), cy(d), cz(d)), d, p.plane);
       }
-      putS<NTS>(&o[i], St::storeStep(f[d], p.scale, (uint32_t)i, p.dither));
+      putS<NTS>(&o[i], St::storeStep(f[d], p.scale, ditherDir(dz, d)));
     }
   }
 }

@@ -55,11 +55,12 @@ __global__ __launch_bounds__(512) void kStepTile(S *a, S *b, const MaskWin *__re
   if (p.addForce)
     addForce(f, p.f1, p.f2);
   collide(f, p.trt);
+  const Dither dz = ditherOf(cell, stepSeq(p), p.dither);
   if constexpr (V == V_PULL) {
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
       const uint64_t i = slot<false>(cell, d, p.plane);
-      b[i] = St::storeStep(f[d], p.scale, (uint32_t)i, p.dither);
+      b[i] = St::storeStep(f[d], p.scale, ditherDir(dz, d));
     }
   } else {
 #pragma unroll
@@ -70,7 +71,7 @@ __global__ __launch_bounds__(512) void kStepTile(S *a, S *b, const MaskWin *__re
         const uint64_t n = nbr(cx(d), cy(d), cz(d), bb);
         i = bb ? slot<false>(cell, inv(d), p.plane) : slot<false>(n, d, p.plane);
       }
-      b[i] = St::storeStep(f[d], p.scale, (uint32_t)i, p.dither);
+      b[i] = St::storeStep(f[d], p.scale, ditherDir(dz, d));
     }
   }
 }
