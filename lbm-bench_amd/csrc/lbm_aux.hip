@@ -460,12 +460,48 @@ __global__ __launch_bounds__(256) void kBuildMask(const int *__restrict__ lat,
   mask[((uint64_t)x * ny + y) * segs + seg] = MaskWin{ lo, hi };
 }
 
+// bit 63 of hi: a cell of the 3x3 rows around the window (z halo included)
+// is solid, so the wavefront of this window needs bounce-back.  Reads the
+// windows from src and writes them with the flag to dst.
+__global__ __launch_bounds__(256) void kMaskNeighbourhood(const MaskWin *__restrict__ src,
+    MaskWin *__restrict__ dst, int nx, int ny, int segs)
+{
+  const uint64_t t     = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t total = (uint64_t)nx * ny * segs;
+  if (t >= total)
+    return;
+  const int x   = (int)(t % nx);
+  const int y   = (int)((t / nx) % ny);
+  const int seg = (int)(t / ((uint64_t)nx * ny));
+  uint64_t any  = 0;
+  for (int ox = -1; ox <= 1; ++ox)
+    for (int oy = -1; oy <= 1; ++oy) {
+      const int xn = wrapAdd(x, ox, nx), yn = wrapAdd(y, oy, ny);
+      const MaskWin w = src[((uint64_t)xn * ny + yn) * segs + seg];
+      any |= w.lo | (w.hi & 3ull);
+    }
+  MaskWin w = src[((uint64_t)x * ny + y) * segs + seg];
+  w.hi |= (any != 0 ? 1ull : 0ull) << 63;
+  dst[((uint64_t)x * ny + y) * segs + seg] = w;
+}
+
 void launchBuildMask(const int *lattice, MaskWin *mask, int nx, int ny, int nz, int segs,
     hipStream_t s)
 {
   const uint64_t total  = (uint64_t)nx * ny * segs;
   const unsigned blocks = (unsigned)((total + 255) / 256);
-  hipLaunchKernelGGL(kBuildMask, dim3(blocks), dim3(256), 0, s, lattice, mask, nx, ny, nz, segs);
+  MaskWin *tmp          = nullptr;
+  if (hipMalloc(&tmp, total * sizeof(MaskWin)) != hipSuccess) {
+    fprintf(stderr, "ERROR: hipMalloc of %zu bytes for the mask windows failed\n",
+        (size_t)(total * sizeof(MaskWin)));
+    exit(EXIT_FAILURE);
+  }
+  hipLaunchKernelGGL(kBuildMask, dim3(blocks), dim3(256), 0, s, lattice, tmp, nx, ny, nz, segs);
+  hipLaunchKernelGGL(kMaskNeighbourhood, dim3(blocks), dim3(256), 0, s, tmp, mask, nx, ny, segs);
+  if (hipStreamSynchronize(s) != hipSuccess || hipFree(tmp) != hipSuccess) {
+    fprintf(stderr, "ERROR: building the mask windows failed\n");
+    exit(EXIT_FAILURE);
+  }
 }
 
 } // namespace lbm

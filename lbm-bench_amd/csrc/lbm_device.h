@@ -8,7 +8,9 @@
 //   AoS    element (d, cell) at cell*19 + d
 //   mask   one MaskWin per (row = x*nY + y, seg = z/64): 66 solid bits
 //          w[k] = solid(x, y, (64*seg - 1 + k) mod nZ), k = 0..65,
-//          stored as lo = w[1..64], hi bit0 = w[0], hi bit1 = w[65].
+//          stored as lo = w[1..64], hi bit0 = w[0], hi bit1 = w[65];
+//          hi bit 63 flags a solid cell among the windows of the 3x3
+//          rows around (x, y) (bounce-back needed by this wavefront).
 //          Lane l of the wavefront that owns z = 64*seg + l reads its
 //          z-1, z, z+1 bits as w[l], w[l+1], w[l+2].
 // Periodicity is index wrap-around on every axis; a non-periodic axis has
@@ -204,9 +206,13 @@ struct Store<__hip_bfloat16> {
 };
 
 // ---- solid mask -----------------------------------------------------------
+// lo: bit l = cell z0 + l of the row segment; hi: bit 0 = z0 - 1, bit 1 =
+// z0 + 64 (both wrapped), bit 63 = some cell of the 3x3 rows around the
+// segment (z halo included) is solid
 struct alignas(16) MaskWin {
   uint64_t lo, hi;
 };
+__device__ __forceinline__ bool nearSolid(MaskWin w) { return (w.hi >> 63) != 0; }
 
 // window shifted so that bit l is the z-1 / z / z+1 neighbour of lane l
 struct RowBits {

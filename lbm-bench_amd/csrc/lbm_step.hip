@@ -99,9 +99,250 @@ __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const
   }
 }
 
+// ---- row sweep: aligned scatter of the z-shifted directions ---------------------
+// The push and AA odd steps write f*_d to x + c_d.  For the ten directions
+// with c_z != 0 that target is one element off the wavefront's 64-element
+// block, so a plain store leaves two partial 128-byte lines per block and
+// direction, which HBM completes by read-modify-write.  Here one wavefront
+// owns a whole (x, y) row (nZ a multiple of 64, planes below 2^32
+// elements) and walks its blocks in z
+// order.  Lane l of the store for block k writes element 64k + l of the
+// target row, taking the value of lane l - c_z through a one-lane DPP shift:
+//   c_z = +1: lane 0 takes lane 63 of block k - 1 (carried as a scalar);
+//             block 0's element 0 waits for block nZ/64 - 1 and is stored
+//             alone after the loop (one partial line per row).
+//   c_z = -1: lane 63 takes lane 0 of block k + 1, so block k is stored one
+//             iteration later; the last block takes lane 0 of block 0,
+//             saved as a scalar.
+// A streamed store is masked where the producing cell or the target cell
+// is solid; bounce-back stores go to the producer's own slot, as in
+// kStepRow.  Every slot keeps a single writer (the cell that reads it in
+// the AA odd step, or the push source), so the delayed stores do not race.
+template <typename S>
+__device__ __forceinline__ S laneShift(S v, S fill, bool up)
+{
+  // up: lane l <- lane l - 1 (wave_shr:1), lane 0 <- fill
+  // down: lane l <- lane l + 1 (wave_shl:1), lane 63 <- fill
+  if constexpr (sizeof(S) == 8) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v), f = __builtin_bit_cast(uint64_t, fill);
+    const int lo = up ? __builtin_amdgcn_update_dpp((int)(uint32_t)f, (int)(uint32_t)u, 0x138, 0xf, 0xf, false)
+                      : __builtin_amdgcn_update_dpp((int)(uint32_t)f, (int)(uint32_t)u, 0x130, 0xf, 0xf, false);
+    const int hi = up ? __builtin_amdgcn_update_dpp((int)(uint32_t)(f >> 32), (int)(uint32_t)(u >> 32), 0x138, 0xf, 0xf, false)
+                      : __builtin_amdgcn_update_dpp((int)(uint32_t)(f >> 32), (int)(uint32_t)(u >> 32), 0x130, 0xf, 0xf, false);
+    return __builtin_bit_cast(S, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+  } else if constexpr (sizeof(S) == 4) {
+    const int u = __builtin_bit_cast(int, v), f = __builtin_bit_cast(int, fill);
+    const int r = up ? __builtin_amdgcn_update_dpp(f, u, 0x138, 0xf, 0xf, false)
+                     : __builtin_amdgcn_update_dpp(f, u, 0x130, 0xf, 0xf, false);
+    return __builtin_bit_cast(S, r);
+  } else {
+    const int u = (int)__builtin_bit_cast(unsigned short, v);
+    const int f = (int)__builtin_bit_cast(unsigned short, fill);
+    const int r = up ? __builtin_amdgcn_update_dpp(f, u, 0x138, 0xf, 0xf, false)
+                     : __builtin_amdgcn_update_dpp(f, u, 0x130, 0xf, 0xf, false);
+    return __builtin_bit_cast(S, (unsigned short)r);
+  }
+}
+
+// value the compiler must treat as unknown at this point (no hoisting or CSE)
+template <typename T>
+__device__ __forceinline__ T opaqueS(T v)
+{
+  asm volatile("" : "+s"(v));
+  return v;
+}
+
+// value of lane `lane` as a wavefront-uniform value
+template <typename S>
+__device__ __forceinline__ S laneValue(S v, int lane)
+{
+  if constexpr (sizeof(S) == 8) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return __builtin_bit_cast(S, ((uint64_t)hi << 32) | lo);
+  } else if constexpr (sizeof(S) == 4) {
+    return __builtin_bit_cast(S, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+  } else {
+    const int r = __builtin_amdgcn_readlane((int)__builtin_bit_cast(unsigned short, v), lane);
+    return __builtin_bit_cast(S, (unsigned short)r);
+  }
+}
+
+template <int V, int NT, typename S>
+__global__ __launch_bounds__(256, LBM_MIN_WAVES) void kRowSweep(S *a, S *b,
+    const MaskWin *__restrict__ mask, Params<S> p, int remap)
+{
+  using R  = typename Store<S>::R;
+  using St = Store<S>;
+  constexpr bool NTS = NT & 1, NTL = NT & 2;
+  static_assert(V == V_PUSH || V == V_AA_ODD, "row sweep: push and AA odd only");
+
+  const unsigned blk = remap ? xcdLogicalBlock(blockIdx.x, gridDim.x) : blockIdx.x;
+  const int row = __builtin_amdgcn_readfirstlane((int)(blk * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+  if (row >= p.nrows)
+    return;
+  const int lane = (int)(threadIdx.x & 63u);
+  const int nz = p.nz, K = p.segs;
+  const int x = row / p.ny, y = row - x * p.ny;
+  const int xs[3] = { wrapDec(x, p.nx), x, wrapInc(x, p.nx) };
+  const int ys[3] = { wrapDec(y, p.ny), y, wrapInc(y, p.ny) };
+  auto rowOf = [&](int ox, int oy) -> uint64_t {
+    return (uint64_t)xs[ox + 1] * (uint64_t)p.ny + (uint64_t)ys[oy + 1];
+  };
+  // Slot and row offsets are formed at each use from strides the compiler
+  // cannot see through: kept live across the block loop, the ~40 distinct
+  // 64-bit offsets exceed the SGPRs and spill.
+  // (the launcher guarantees plane < 2^32, so cell indices are 32-bit)
+  auto sl = [&](uint64_t c, int d) -> uint64_t {
+    return c + (uint64_t)(uint32_t)d * (uint64_t)opaqueS((uint32_t)p.plane);
+  };
+  auto rowStart = [&](int ox, int oy) -> uint64_t {
+    const uint32_t r = (uint32_t)xs[ox + 1] * (uint32_t)p.ny + (uint32_t)ys[oy + 1];
+    return (uint64_t)(r * opaqueS((uint32_t)nz));
+  };
+  auto winOf = [&](int ox, int oy, int k) -> MaskWin { return mask[rowOf(ox, oy) * K + k]; };
+  auto bitsOf = [&](int ox, int oy, int oz, int k) -> uint64_t {
+    const RowBits rb = rowBits(winOf(ox, oy, k));
+    return oz < 0 ? rb.m : (oz > 0 ? rb.p : rb.c);
+  };
+  // lanes of block k whose streamed store to row (ox, oy) is skipped: the
+  // producer (z - oz) or the target (z) is solid
+  auto skipOf = [&](int ox, int oy, int oz, int k) -> uint64_t {
+    return bitsOf(0, 0, -oz, k) | bitsOf(ox, oy, 0, k);
+  };
+  S *o = V == V_PUSH ? b : a;
+
+  S carry[Q], pend[Q], first[Q];
+#pragma unroll
+  for (int d = 0; d < Q; ++d) {
+    carry[d] = S{};
+    pend[d]  = S{};
+    first[d] = S{};
+  }
+  bool prevClean = true; // block k - 1 had no solid cell near it
+
+  for (int k = 0; k < K; ++k) {
+    const int z  = k * 64 + lane;
+    const int zm = z == 0 ? nz - 1 : z - 1, zp = z == nz - 1 ? 0 : z + 1;
+    const uint64_t cell = rowStart(0, 0) + (uint64_t)z;
+    const MaskWin wc    = winOf(0, 0, k);
+    auto cellN = [&](int ox, int oy, int oz) -> uint64_t {
+      return rowStart(ox, oy) + (uint64_t)(oz < 0 ? zm : (oz > 0 ? zp : z));
+    };
+
+    // One block.  walls = false: no cell of the 3x3 rows around the block
+    // (z halo included) is solid, so there is no bounce-back and no store
+    // to skip.
+    auto block = [&](auto walls) {
+      constexpr bool W = decltype(walls)::value;
+      const bool fluid = W ? !__builtin_amdgcn_inverse_ballot_w64(wc.lo) : true;
+      R f[Q];
+      if (fluid) {
+#pragma unroll
+        for (int d = 0; d < Q; ++d) {
+          uint64_t i = sl(cell, d);
+          if (V == V_AA_ODD && d != C) {
+            i = sl(cellN(-cx(d), -cy(d), -cz(d)), inv(d));
+            if (W && __builtin_amdgcn_inverse_ballot_w64(bitsOf(-cx(d), -cy(d), -cz(d), k)))
+              i = sl(cell, d);
+          }
+          f[d] = St::load(getS<NTL>(&a[i]), p.inv);
+        }
+      } else {
+#pragma unroll
+        for (int d = 0; d < Q; ++d)
+          f[d] = R(0);
+      }
+      if (p.addForce)
+        addForce(f, p.f1, p.f2);
+      collide(f, p.trt);
+      const Dither dz = ditherOf(cell, stepSeq(p), p.dither);
+
+#pragma unroll
+      for (int d = 0; d < Q; ++d) {
+        const S v = St::storeStep(f[d], p.scale, ditherDir(dz, d));
+        if (d == C) {
+          if (fluid)
+            putS<NTS>(&o[sl(cell, C)], v);
+          continue;
+        }
+        if (cz(d) == 0) {
+          uint64_t i = sl(cellN(cx(d), cy(d), 0), d);
+          if (W && __builtin_amdgcn_inverse_ballot_w64(bitsOf(cx(d), cy(d), 0, k)))
+            i = sl(cell, inv(d)); // bounce-back: own slot inv d
+          if (fluid)
+            putS<NTS>(&o[i], v);
+          continue;
+        }
+        if (W && fluid && __builtin_amdgcn_inverse_ballot_w64(bitsOf(cx(d), cy(d), cz(d), k)))
+          putS<NTS>(&o[sl(cell, inv(d))], v); // bounce-back
+        const uint64_t tgt = rowStart(cx(d), cy(d));
+        if (cz(d) > 0) {
+          const S sh = laneShift(v, carry[d], true);
+          carry[d]   = laneValue(v, 63);
+          uint64_t skip = W ? skipOf(cx(d), cy(d), 1, k) : 0ull;
+          if (k == 0)
+            skip |= 1ull; // element 0 comes from the last block
+          if (!__builtin_amdgcn_inverse_ballot_w64(skip))
+            putS<NTS>(&o[sl(tgt + (uint64_t)z, d)], sh);
+        } else {
+          if (k == 0) {
+            first[d] = laneValue(v, 0);
+          } else {
+            // block k - 1, completed by lane 0 of this block
+            const S done = laneShift(pend[d], laneValue(v, 0), false);
+            const uint64_t skip = prevClean ? 0ull : skipOf(cx(d), cy(d), -1, k - 1);
+            if (!__builtin_amdgcn_inverse_ballot_w64(skip))
+              putS<NTS>(&o[sl(tgt + (uint64_t)(z - 64), d)], done);
+          }
+          pend[d] = v;
+        }
+      }
+    };
+    const bool clean = !nearSolid(wc);
+    if (clean)
+      block(std::false_type{});
+    else
+      block(std::true_type{});
+    prevClean = clean;
+  }
+
+  // the wrap-around: element 0 (c_z = +1) and the last block (c_z = -1)
+#pragma unroll
+  for (int d = 0; d < Q; ++d) {
+    if (cz(d) == 0)
+      continue;
+    const uint64_t tgt = rowStart(cx(d), cy(d));
+    if (cz(d) > 0) {
+      const uint64_t skip = skipOf(cx(d), cy(d), 1, 0) | ~1ull;
+      if (!__builtin_amdgcn_inverse_ballot_w64(skip))
+        putS<NTS>(&o[sl(tgt, d)], carry[d]);
+    } else {
+      const S done        = laneShift(pend[d], first[d], false);
+      const uint64_t skip = prevClean ? 0ull : skipOf(cx(d), cy(d), -1, K - 1);
+      if (!__builtin_amdgcn_inverse_ballot_w64(skip))
+        putS<NTS>(&o[sl(tgt + (uint64_t)((K - 1) * 64 + lane), d)], done);
+    }
+  }
+}
+
+template <int V, int NT, typename S>
+static void launchRowSweep(const StepLaunch &L, hipStream_t s)
+{
+  Params<S> p           = makeParams<S>(L);
+  const unsigned blocks = (unsigned)(((uint64_t)L.g.nx * L.g.ny + 3) / 4);
+  hipLaunchKernelGGL((kRowSweep<V, NT, S>), dim3(blocks), dim3(256), 0, s, (S *)L.src,
+      (S *)L.dst, L.g.mask, p, L.remap);
+}
+
 template <int V, bool AOS, int NT, typename S>
 static void launchT(const StepLaunch &L, hipStream_t s)
 {
+  if constexpr (!AOS && (V == V_PUSH || V == V_AA_ODD)) {
+    if (L.rowSweep && L.g.nz % 64 == 0 && L.g.plane < (1ull << 32))
+      return launchRowSweep<V, NT, S>(L, s);
+  }
   Params<S> p       = makeParams<S>(L);
   const int wpb     = L.wavesPerBlock == 8 ? 8 : 4;
   const unsigned blocks = rowBlocks(L.g, wpb);

@@ -49,6 +49,12 @@ CASES = [
     ("box", (10, 11, 12), (1, 0, 1)),
     ("pipe", (12, 12, 12), (0, 0, 0)),
     ("spheres-6", (20, 22, 70), (0, 0, 1)),
+    # nZ a multiple of 64 with several 64-blocks per row: the row-sweep
+    # push and AA odd kernels (aligned scatter, carries between blocks and
+    # the z wrap), with solids inside the rows, walls, and no solids at all
+    ("spheres-6", (16, 18, 128), (0, 0, 1)),
+    ("box", (10, 11, 192), (1, 0, 1)),
+    ("fluid", (6, 7, 192), (1, 1, 1)),
 ]
 
 
@@ -56,8 +62,8 @@ CASES = [
 @pytest.mark.parametrize("kernel", KERNELS)
 def test_matches_oracle(gpu_lib, oracle, geo, dims, per, kernel):
     """Lambda 3/16, with and without the force, an odd step count; the
-    spheres case has nZ = 70 (a partial 64-wide segment) and wraps spheres
-    through the periodic faces"""
+    first spheres case has nZ = 70 (a partial 64-wide segment) and wraps
+    spheres through the periodic faces"""
     lat, per_o, _ = oracle.geometry(geo, dims, per, 3)
     init = oracle.init_random(dims, lat, 17, 1e-2)
     fluid = lat.transpose(2, 1, 0) != 0
