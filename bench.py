@@ -105,6 +105,88 @@ def adjoint_run(lib, lbmhip):
     return out
 
 
+def study_precision(lib, lbmhip, n, spheres, total, chunk):
+    """The 512^3 spheres workload in FP64, FP32, FP16 and BF16 storage, run
+    side by side for `total` steps; after every `chunk` steps the velocity
+    error and mass drift of each form against the FP64 run"""
+    dims = (n, n, n)
+    kw = dict(omega=1.25, xforce=1e-6, body_force=True, magic=3.0 / 16.0, seed=0)
+    geo = f"spheres-{spheres}"
+    lats = {form: lbmhip.Lattice(lib, "aa-soa", geo, dims, (0, 0, 1), storage=form, **kw)
+            for form in ("native", "fp32", "fp16", "bf16")}
+    for L in lats.values():
+        L.init("random", 0, 1.0, 1e-3)
+    m0 = {form: L.mass_momentum() for form, L in lats.items()}
+    nfluid = lats["native"].n_fluid
+    secs = {form: 0.0 for form in lats}
+    rows = []
+    for done in range(chunk, total + 1, chunk):
+        for form, L in lats.items():
+            secs[form] += L.run(chunk)
+        row = {"steps": done}
+        for form, L in lats.items():
+            m = L.mass_momentum()
+            entry = {"mass_drift_rel": (m[0] - m0[form][0]) / m0[form][0]}
+            if form != "native":
+                cmp = L.compare_momentum(lats["native"])
+                entry["velocity_l2_rel_err_vs_fp64"] = (cmp[0] / cmp[1]) ** 0.5
+                entry["velocity_max_abs_err_vs_fp64"] = cmp[2]
+            row["fp64" if form == "native" else form] = entry
+        rows.append(row)
+        log(f"precision study: {done} steps, u L2 err fp32 "
+            f"{row['fp32']['velocity_l2_rel_err_vs_fp64']:.3e} fp16 "
+            f"{row['fp16']['velocity_l2_rel_err_vs_fp64']:.3e} bf16 "
+            f"{row['bf16']['velocity_l2_rel_err_vs_fp64']:.3e}")
+    for L in lats.values():
+        L.close()
+    return {"study": "precision forms against FP64", "workload": f"aa-soa {n}^3 channel with "
+            f"{spheres} seeded spheres, omega 1.25, Lambda 3/16, body force 1e-6, "
+            f"f = w(1+eps)", "fluid_cells": nfluid,
+            "mlups": {("fp64" if f == "native" else f): round(nfluid * total / secs[f] / 1e6, 1)
+                      for f in secs},
+            "series": rows}
+
+
+def study_adjoint(lib, lbmhip, n, T, intervals, sigma):
+    """The differentiation form at scale: an n^3 porous channel (s from
+    U[0,1] noise smoothed with sigma and a logistic), FP16 forward storage
+    with an FP32 adjoint, multi-level checkpoints; J also from an FP64
+    forward run of the same case"""
+    dims = (n, n, n)
+    s = lbmhip.porous_field(lib, dims, 0, sigma)
+    kw = dict(omega=1.25, magic=3.0 / 16.0, xforce=1e-5)
+    A = lbmhip.Adjoint(lib, "channel", dims, (0, 0, 1), storage="fp16", **kw)
+    nf = int(A.fluid_mask().sum())
+    A.set_field(s)
+    A.init("rest", 0, 1.0, 0.0)
+    st = A.step_times(10)
+    log(f"adjoint study: {n}^3, T {T}, checkpoints {intervals}: forward step "
+        f"{st['forward_s'] * 1e3:.3f} ms, adjoint step {st['adjoint_s'] * 1e3:.3f} ms")
+    J, g, stats = A.gradient(T, intervals)
+    A.close()
+    total = stats["forward_s"] + stats["recompute_s"] + stats["adjoint_s"]
+    log(f"adjoint study: gradient in {total:.1f} s, peak {stats['device_bytes'] / 2**30:.1f} GiB")
+    B = lbmhip.Adjoint(lib, "channel", dims, (0, 0, 1), storage="native", **kw)
+    B.set_field(s)
+    B.init("rest", 0, 1.0, 0.0)
+    J64, t64 = B.forward(T)
+    B.close()
+    return {"study": "adjoint at scale", "workload": f"Brinkman TRT {n}^3 porous channel "
+            f"(s: U[0,1] noise, Gaussian sigma {sigma}, logistic 4z-2), body force 1e-5, rest "
+            f"start, T {T}, J = outlet-plane kinetic energy", "fluid_cells": nf,
+            "storage": "FP16 forward, FP32 adjoint", "checkpoint_intervals": intervals,
+            "J": J, "J_fp64_forward": J64, "J_rel_diff": abs(J - J64) / abs(J64),
+            "forward_step_ms": round(st["forward_s"] * 1e3, 4),
+            "adjoint_step_ms": round(st["adjoint_s"] * 1e3, 4),
+            "adjoint_to_forward_step": round(st["adjoint_s"] / st["forward_s"], 3),
+            "forward_s": round(stats["forward_s"], 3), "recompute_s": round(stats["recompute_s"], 3),
+            "adjoint_s": round(stats["adjoint_s"], 3),
+            "mlups_gradient": round(nf * T / total / 1e6, 1),
+            "gradient_to_forward_time": round(total / stats["forward_s"], 3),
+            "peak_device_gib": round(stats["device_bytes"] / 2 ** 30, 2),
+            "gradient_l2_norm": float(np.linalg.norm(g))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +198,10 @@ def main():
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--study", choices=["precision5000", "adjoint384"],
+                    help="instead of the bench line, one JSON line of a longer study: the four "
+                         "storage forms over 5000 steps, or the 384^3 FP16/FP32 adjoint with "
+                         "T = 4096 and two checkpoint levels")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -127,6 +213,14 @@ def main():
     lib = lbmhip.Library("dp")
     dev, cus, mem = lib.device_info()
     log(f"device {dev}, {cus} CUs, {mem / 2**30:.0f} GiB")
+
+    if args.study == "precision5000":
+        print(json.dumps(study_precision(lib, lbmhip, args.size, args.spheres, 5000, 1000)),
+              flush=True)
+        return
+    if args.study == "adjoint384":
+        print(json.dumps(study_adjoint(lib, lbmhip, 384, 4096, [64, 8], 5.0)), flush=True)
+        return
 
     n = args.size
     dims = (n, n, n)
