@@ -115,6 +115,8 @@ class Library:
         L.kernelSetInitialDensity.argtypes = [LDP, KDP, CDP]
         L.kernelSetInitialVelocity.argtypes = [LDP, KDP, CDP]
         L.kernelAddBodyForce.argtypes = [KDP, LDP, CDP]
+        L.kernelComputeBoundaryConditions.argtypes = [KDP, LDP, CDP]
+        L.kernelComputeBoundaryConditions.restype = None
         L.lbmAdjointCreate.argtypes = [LDP, CDP, c_int]
         L.lbmAdjointCreate.restype = c_void_p
         L.lbmAdjointDestroy.argtypes = [c_void_p]
@@ -198,6 +200,22 @@ class Lattice:
     def fluid_mask(self):
         """bool (nX, nY, nZ): True on non-obstacle cells"""
         return self.lattice_flags().transpose(2, 1, 0) != 0
+
+    def set_flags(self, flags):
+        """relabel non-obstacle cells (fluid / inlet / outlet, LatticeDescType
+        flags, (nZ, nY, nX)); which cells are obstacles is fixed at Init"""
+        flags = np.ascontiguousarray(flags, dtype=np.int32)
+        if ((flags == 0) != (self.lattice_flags() == 0)).any():
+            raise ValueError("set_flags: the obstacle cells cannot change after Init")
+        n = self.dims[0] * self.dims[1] * self.dims[2]
+        np.ctypeslib.as_array(self.ld.Lattice, shape=(n,))[:] = flags.reshape(-1)
+
+    def boundary_conditions(self, rho_in, rho_out):
+        """kernelComputeBoundaryConditions: Zou-He pressure inlet at x = 0 and
+        outlet at x = nX-1 on the cells flagged inlet / outlet"""
+        self.cd.RhoIn, self.cd.RhoOut = rho_in, rho_out
+        self.lib.lib.kernelComputeBoundaryConditions(self.kd, ctypes.byref(self.ld),
+                                                     ctypes.byref(self.cd))
 
     @property
     def n_fluid(self):

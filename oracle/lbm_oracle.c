@@ -26,7 +26,10 @@
  * reduces TRT to BGK, and Lambda = 3/16 makes steady Poiseuille flow exact
  * (tests/test_oracle_closed_forms.py).  The "spheres-N" geometry and the
  * seeded initialiser have no reference counterpart; they are specified
- * here and pinned by brute-force restatements in the tests.
+ * here and pinned by brute-force restatements in the tests.  The Zou-He
+ * boundaries are pinned against the reference's own
+ * kernelComputeBoundaryConditions (oracle/_ref) after runs of every
+ * variant.
  */
 #include "lbm_oracle.h"
 
@@ -984,4 +987,51 @@ void oracle_porous_field(const int dims[3], uint64_t seed, double sigma, double 
   free(u);
   free(t);
   free(w);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Zou-He pressure boundaries (src/lbm.c:676-785)                           */
+/* ------------------------------------------------------------------------ */
+/* On the logical PDFs: at every inlet cell of the plane x = 0 (flag
+ * LAT_CELL_INLET) the density is set to rhoIn by the five populations
+ * entering along +x; at every outlet cell of x = nX-1 (LAT_CELL_OUTLET) to
+ * rhoOut by the five entering along -x.  y and z run over 1 .. n-2, as in
+ * the reference, and the arithmetic keeps its grouping (rho^-1 as a
+ * multiplier, the 1/4 and 1/6 factors, the sum order). */
+void oracle_zou_he(const int dims[3], const int *lat, double *pdfs, double rhoIn, double rhoOut)
+{
+  const int nX = dims[0], nY = dims[1], nZ = dims[2];
+  const double inInv = 1.0 / rhoIn, outInv = 1.0 / rhoOut;
+  const double third = 1.0 / 3.0, fourth = 1.0 / 4.0, sixth = 1.0 / 6.0;
+  for (int z = 1; z < nZ - 1; ++z)
+    for (int y = 1; y < nY - 1; ++y) {
+      if (lat[((size_t)z * nY + y) * nX + 0] == 2) { /* LAT_CELL_INLET */
+        double *f = pdfs + (((size_t)0 * nY + y) * nZ + z) * Q;
+        const double rho = rhoIn;
+        const double ux  = 1.0 - (f[C] + (f[T] + f[B] + f[S] + f[N]) +
+                                     (f[TS] + f[BS] + f[TN] + f[BN]) +
+                                     2.0 * (f[SW] + f[TW] + f[W] + f[BW] + f[NW])) *
+                                    inInv;
+        const double ind = sixth * rho * ux;
+        f[E]  = f[W] + third * rho * ux;
+        f[NE] = f[SW] - fourth * (f[N] - f[S]) + ind;
+        f[SE] = f[NW] + fourth * (f[N] - f[S]) + ind;
+        f[TE] = f[BW] - fourth * (f[T] - f[B]) + ind;
+        f[BE] = f[TW] + fourth * (f[T] - f[B]) + ind;
+      }
+      if (lat[((size_t)z * nY + y) * nX + (nX - 1)] == 4) { /* LAT_CELL_OUTLET */
+        double *f = pdfs + (((size_t)(nX - 1) * nY + y) * nZ + z) * Q;
+        const double rho = rhoOut;
+        const double ux  = -1.0 + (f[C] + (f[T] + f[B] + f[S] + f[N]) +
+                                      (f[TS] + f[BS] + f[TN] + f[BN]) +
+                                      2.0 * (f[NE] + f[BE] + f[E] + f[TE] + f[SE])) *
+                                     outInv;
+        const double ind = sixth * rho * ux;
+        f[W]  = f[E] - third * rho * ux;
+        f[SW] = f[NE] + fourth * (f[N] - f[S]) - ind;
+        f[NW] = f[SE] - fourth * (f[N] - f[S]) - ind;
+        f[BW] = f[TE] + fourth * (f[T] - f[B]) - ind;
+        f[TW] = f[BE] - fourth * (f[T] - f[B]) - ind;
+      }
+    }
 }

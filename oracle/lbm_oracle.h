@@ -44,6 +44,10 @@ double oracle_brinkman(const int dims[3], const int *lattice, const double *s,
     double *final_state, double *grad);
 
 /* porous solid-fraction field (smoothed, standardised, logistic) */
+/* Zou-He pressure inlet/outlet on logical PDFs, in place (src/lbm.c:676-785) */
+void oracle_zou_he(const int dims[3], const int *lattice, double *pdfs, double rhoIn,
+    double rhoOut);
+
 void oracle_porous_field(const int dims[3], uint64_t seed, double sigma, double a, double b,
     double *s);
 

@@ -60,6 +60,8 @@ class Oracle:
         L.oracle_brinkman.argtypes = [_I3, _ip, _dp, _dp, ctypes.c_int, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp]
         L.oracle_brinkman.restype = ctypes.c_double
+        L.oracle_zou_he.argtypes = [_I3, _ip, _dp, ctypes.c_double, ctypes.c_double]
+        L.oracle_zou_he.restype = None
         L.oracle_porous_field.argtypes = [_I3, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
                                           ctypes.c_double, _dp]
         L.oracle_porous_field.restype = None
@@ -135,6 +137,14 @@ class Oracle:
                                      _ptr(grad, _dp) if gradient else None)
         return J, final, grad
 
+    def zou_he(self, dims, lattice, pdfs, rho_in, rho_out):
+        """Zou-He inlet (x = 0, flag 2) / outlet (x = nX-1, flag 4) on a copy
+        of the logical PDFs (src/lbm.c:676-785)"""
+        out = np.ascontiguousarray(pdfs, dtype=np.float64).copy()
+        lat = np.ascontiguousarray(lattice, dtype=np.int32)
+        self.lib.oracle_zou_he(_I3(*dims), _ptr(lat, _ip), _ptr(out, _dp), rho_in, rho_out)
+        return out
+
     def porous_field(self, dims, seed, sigma, a=4.0, b=-2.0):
         out = np.zeros(tuple(dims))
         self.lib.oracle_porous_field(_I3(*dims), seed, sigma, a, b, _ptr(out, _dp))
@@ -153,6 +163,10 @@ class Reference:
         L.ref_run.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _I3, _I3, _dp, ctypes.c_int,
                               ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp]
         L.ref_run.restype = ctypes.c_int
+        L.ref_zou_he.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _I3, _I3, _dp, ctypes.c_int,
+                                 ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_double, _dp]
+        L.ref_zou_he.restype = ctypes.c_int
         L.ref_verification_norm.argtypes = [ctypes.c_char_p]
         L.ref_verification_norm.restype = ctypes.c_double
 
@@ -183,6 +197,19 @@ class Reference:
         if rc != 0:
             raise ValueError(f"unknown kernel {kernel}")
         return out, tuple(stats)
+
+    def zou_he(self, kernel, geometry, dims, periodic, init, iterations, omega, xforce,
+               rho_in, rho_out):
+        """a run, then the x faces marked inlet/outlet and the reference's
+        kernelComputeBoundaryConditions once -> logical pdfs"""
+        out = np.zeros(tuple(dims) + (19,), dtype=np.float64)
+        init = np.ascontiguousarray(init, dtype=np.float64)
+        rc = self.lib.ref_zou_he(kernel.encode(), geometry.encode(), _I3(*dims), _I3(*periodic),
+                                 _ptr(init, _dp), iterations, omega, xforce, rho_in, rho_out,
+                                 _ptr(out, _dp))
+        if rc != 0:
+            raise ValueError(f"unknown kernel {kernel}")
+        return out
 
     def verification_norm(self, kernel):
         return self.lib.ref_verification_norm(kernel.encode())

@@ -51,9 +51,10 @@ int ref_geometry(const char *type, const int dims_in[3], int periodic_io[3],
  *             obstacle cells are written as 0).
  *   stats     {kernelDensity, kernelVerify error norm, Duration}.
  * Returns 0, or -1 for an unknown kernel name. */
-int ref_run(const char *kernel, const char *geometry, const int dims_in[3],
+static int runCase(const char *kernel, const char *geometry, const int dims_in[3],
     const int periodic_in[3], const double *init, int ref_init, int iterations,
-    double omega, double xforce, double *out, double stats[3])
+    double omega, double xforce, int zouHe, double rhoIn, double rhoOut, double *out,
+    double stats[3])
 {
   int dims[3]     = { dims_in[0], dims_in[1], dims_in[2] };
   int periodic[3] = { periodic_in[0], periodic_in[1], periodic_in[2] };
@@ -64,8 +65,8 @@ int ref_run(const char *kernel, const char *geometry, const int dims_in[3],
 
   CaseDataType cd;
   cd.MaxIterations     = iterations;
-  cd.RhoIn             = F(1.0);
-  cd.RhoOut            = F(1.0);
+  cd.RhoIn             = (PdfType)rhoIn;
+  cd.RhoOut            = (PdfType)rhoOut;
   cd.Omega             = (PdfType)omega;
   cd.XForce            = (PdfType)xforce;
   cd.StatisticsModulus = 100;
@@ -97,6 +98,21 @@ int ref_run(const char *kernel, const char *geometry, const int dims_in[3],
 
   kd->Kernel(&ld, kd, &cd);
 
+  if (zouHe) {
+    /* the non-obstacle cells of the planes x = 0 and x = nX-1 become inlet
+     * and outlet cells, then the reference's boundary pass runs once */
+    for (int z = 1; z < dims[2] - 1; ++z)
+      for (int y = 1; y < dims[1] - 1; ++y) {
+        int *in  = &ld.Lattice[latticeIndex(ld.Dims, 0, y, z)];
+        int *out_ = &ld.Lattice[latticeIndex(ld.Dims, dims[0] - 1, y, z)];
+        if (*in != LAT_CELL_OBSTACLE)
+          *in = LAT_CELL_INLET;
+        if (*out_ != LAT_CELL_OBSTACLE)
+          *out_ = LAT_CELL_OUTLET;
+      }
+    kernelComputeBoundaryConditions(kd, &ld, &cd);
+  }
+
   for (int x = 0; x < nX; ++x)
     for (int y = 0; y < nY; ++y)
       for (int z = 0; z < nZ; ++z) {
@@ -120,6 +136,27 @@ int ref_run(const char *kernel, const char *geometry, const int dims_in[3],
   kf->Deinit(&ld, &kd);
   free(ld.Lattice);
   return 0;
+}
+
+int ref_run(const char *kernel, const char *geometry, const int dims_in[3],
+    const int periodic_in[3], const double *init, int ref_init, int iterations,
+    double omega, double xforce, double *out, double stats[3])
+{
+  return runCase(kernel, geometry, dims_in, periodic_in, init, ref_init, iterations, omega,
+      xforce, 0, 1.0, 1.0, out, stats);
+}
+
+/* As ref_run without ref_init, then the non-obstacle cells of x = 0 and
+ * x = nX-1 (y, z in 1 .. n-2) are marked inlet and outlet and the
+ * reference's kernelComputeBoundaryConditions (src/lbm.c:676-785) runs once
+ * with rhoIn / rhoOut before the logical PDFs are read back. */
+int ref_zou_he(const char *kernel, const char *geometry, const int dims_in[3],
+    const int periodic_in[3], const double *init, int iterations, double omega, double xforce,
+    double rhoIn, double rhoOut, double *out)
+{
+  double stats[3];
+  return runCase(kernel, geometry, dims_in, periodic_in, init, 0, iterations, omega, xforce, 1,
+      rhoIn, rhoOut, out, stats);
 }
 
 /* The reference's -V run (cli.c:84-99 + main.c:112-142): box 16^3,

@@ -60,3 +60,15 @@ def moments(pdfs):
     rho = pdfs.sum(-1)
     j = np.stack([(pdfs * CX).sum(-1), (pdfs * CY).sum(-1), (pdfs * CZ).sum(-1)], -1)
     return rho, j
+
+
+def mark_inlet_outlet(lattice):
+    """the non-obstacle cells of x = 0 / x = nX-1 (y, z in 1 .. n-2) as inlet
+    (2) / outlet (4), as the reference's boundary pass expects them"""
+    lat = lattice.copy()
+    inner = (slice(1, lat.shape[0] - 1), slice(1, lat.shape[1] - 1))
+    face_in, face_out = lat[inner + (0,)], lat[inner + (-1,)]
+    face_in[face_in != 0] = 2
+    face_out[face_out != 0] = 4
+    lat[inner + (0,)], lat[inner + (-1,)] = face_in, face_out
+    return lat

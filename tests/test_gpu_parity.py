@@ -8,7 +8,7 @@ import os
 
 import numpy as np
 import pytest
-from conftest import CX, GOLDEN, WEIGHTS, moments, rel_err
+from conftest import CX, GOLDEN, WEIGHTS, mark_inlet_outlet, moments, rel_err
 
 from lbmhip import KERNELS, Lattice
 
@@ -258,3 +258,27 @@ def test_single_precision_build(gpu_lib_sp):
             # variants spread by 0.7 % (0.04340 ... 0.04371)
             spread = max(ver["sp"].values()) - min(ver["sp"].values())
             assert abs(L.verify() - ver["sp"][kernel]) < 2 * spread, kernel
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_zou_he_matches_oracle(gpu_lib, oracle, kernel):
+    """kernelComputeBoundaryConditions (src/lbm.c:676-785) through the
+    library's boundary accessors after 0 and 5 steps, against the oracle's
+    restatement (pinned to the compiled reference in
+    test_oracle_vs_reference.py)"""
+    dims, per = (12, 11, 10), (1, 0, 1)
+    lat, per_o, _ = oracle.geometry("channel", dims, per)
+    init = oracle.init_random(dims, lat, 21, 1e-2)
+    fluid = lat.transpose(2, 1, 0) != 0
+    marked = mark_inlet_outlet(lat)
+    for steps in (0, 5):
+        with Lattice(gpu_lib, kernel, "channel", dims, per, omega=1.25, xforce=1e-4,
+                     body_force=True) as L:
+            L.set_nodes(init)
+            L.run(steps)
+            L.set_flags(marked)
+            L.boundary_conditions(1.01, 0.99)
+            got = L.get_nodes()
+        run, _ = oracle.run(kernel, dims, lat, per_o, init, steps, 1.25, 1 / 12, 1e-4, True)
+        exp = oracle.zou_he(dims, marked, run, 1.01, 0.99)
+        assert rel_err(got, exp, fluid) < TOL, (kernel, steps)

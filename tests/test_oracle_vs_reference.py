@@ -3,7 +3,7 @@
 to machines without it)."""
 import numpy as np
 import pytest
-from conftest import rel_err
+from conftest import mark_inlet_outlet, rel_err
 
 from oracle import KERNELS, Reference
 
@@ -58,3 +58,23 @@ def test_verification_norm_matches_reference(oracle, ref, kernel):
     assert rel_err(got, exp, fluid) < 1e-13
     norm = oracle.verify(dims, lat, got, 1.0, 1e-5, True)
     assert abs(norm - stats[1]) < 1e-10 * stats[1]
+
+
+
+@pytest.mark.parametrize("geo,dims,per", [("channel", (12, 11, 10), (1, 0, 1)),
+                                          ("box", (9, 10, 11), (1, 0, 0))])
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_zou_he_matches_reference(oracle, ref, geo, dims, per, kernel):
+    """kernelComputeBoundaryConditions (src/lbm.c:676-785) after runs of
+    0, 1, 2 and 5 steps (both AA parities; each variant reads and writes the
+    boundary populations through its own accessors, src/lbm.c:296-396)"""
+    lat, per_o, _ = oracle.geometry(geo, dims, per)
+    init = oracle.init_random(dims, lat, 21, 1e-2)
+    fluid = lat.transpose(2, 1, 0) != 0
+    marked = mark_inlet_outlet(lat)
+    for steps in (0, 1, 2, 5):
+        run, _ = oracle.run(kernel, dims, lat, per_o, init, steps, 1.25, 1 / 12, 1e-4, True)
+        got = oracle.zou_he(dims, marked, run, 1.01, 0.99)
+        exp = ref.zou_he(kernel, geo, dims, per, init, steps, 1.25, 1e-4, 1.01, 0.99)
+        assert rel_err(got, exp, fluid) < 1e-14, (kernel, steps)
+        assert not np.array_equal(got, run)  # the boundary pass changed something
