@@ -44,6 +44,42 @@ void lbmMassMomentum(LatticeDescType *ld, KernelDataType *kd, double out[4]);
 void lbmCompareMomentum(LatticeDescType *ld, KernelDataType *a, KernelDataType *b,
     double out[4]);
 
+/* Per-cell density and momentum: out[((x*nY + y)*nZ + z)*4 + k] holds rho,
+ * jx, jy, jz of the logical PDFs (j is the velocity of the incompressible
+ * model); obstacle cells hold 0.  The velocity field the reference can
+ * only form through a GetNode loop (src/lbm.c:964-1023 does it for one
+ * plane in kernelVerify). */
+void lbmCellMoments(LatticeDescType *ld, KernelDataType *kd, double *out);
+
+/* On-disk state: the logical PDFs and the cell flags in one file, for
+ * resume and external validation.  The reference has no state file.
+ * Layout, little-endian:
+ *   bytes   0-7    magic "LBMSTAT1"
+ *           8-11   uint32 format version (1)
+ *          12-15   uint32 header size (128)
+ *          16-27   int32 Dims[3]
+ *          28-39   int32 Periodic[3]
+ *          40-43   int32 19 (velocities)
+ *          44-47   int32 kd->Iteration when saved
+ *          48-71   float64 Omega, XForce, MagicParam of cd (informational)
+ *          72-79   uint64 F: bytes of the flag section
+ *          80-87   uint64 P: bytes of the PDF section
+ *          88-95   uint64 checksum
+ *          96-127  kernel name, NUL padded
+ *   then   F bytes: int8 LatticeDescType flag per cell in latticeIndex
+ *                   order (x fastest), zero-padded to a multiple of 8
+ *   then   P bytes: float64 logical PDFs [x][y][z][19], as lbmGetNodes
+ * The checksum is FNV-1a over the little-endian 64-bit words of the two
+ * sections (offset basis 0xcbf29ce484222325, prime 0x100000001b3).
+ * lbmSaveState streams the lattice to `path` in x-slabs; lbmLoadState
+ * checks magic, version, dimensions and that the obstacle cells are the
+ * same, then loads the PDFs, so a run resumed from a file continues as the
+ * uninterrupted run would.  Both return 0, or -1 after a message on stderr
+ * (I/O error, mismatch, bad checksum; after a checksum failure the
+ * lattice's PDFs are undefined). */
+int lbmSaveState(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd, const char *path);
+int lbmLoadState(LatticeDescType *ld, KernelDataType *kd, const char *path);
+
 /* Runs `steps` steps (as Kernel with MaxIterations = steps) and returns the
  * hipEvent time of the launches in seconds. */
 double lbmRunSteps(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd, int steps);

@@ -36,12 +36,13 @@ struct CellPos {
   uint64_t cell;
   bool valid;
 };
-__device__ __forceinline__ CellPos cellPos(const Geom &g)
+// the thread's cell among [first, end) (logical order, x slowest); valid
+// for in-range fluid cells
+__device__ __forceinline__ CellPos cellPosIn(const Geom &g, uint64_t first, uint64_t end)
 {
   CellPos c;
-  const uint64_t n = (uint64_t)g.nx * g.ny * g.nz;
-  c.cell           = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  c.valid          = c.cell < n;
+  c.cell  = first + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  c.valid = c.cell < end;
   if (!c.valid)
     return c;
   c.z              = (int)(c.cell % g.nz);
@@ -50,6 +51,10 @@ __device__ __forceinline__ CellPos cellPos(const Geom &g)
   c.x              = (int)(r / g.ny);
   c.valid          = !solidAt(g.mask, g.segs, g.ny, c.x, c.y, c.z);
   return c;
+}
+__device__ __forceinline__ CellPos cellPos(const Geom &g)
+{
+  return cellPosIn(g, 0, (uint64_t)g.nx * g.ny * g.nz);
 }
 
 // ---- body force pass -------------------------------------------------------
@@ -165,13 +170,12 @@ void launchInit(const AuxLaunch &A, int mode, uint64_t seed, double rho, double 
 // ---- logical gather / scatter ------------------------------------------------
 template <typename S>
 __global__ __launch_bounds__(256) void kGather(const S *a, Geom g, int rule, double rhoRef,
-    float inv, double *out)
+    float inv, double *out, uint64_t first, uint64_t end)
 {
   using R  = typename Store<S>::R;
   using St = Store<S>;
-  const CellPos c  = cellPos(g);
-  const uint64_t n = (uint64_t)g.nx * g.ny * g.nz;
-  if (c.cell >= n)
+  const CellPos c = cellPosIn(g, first, end);
+  if (c.cell >= end)
     return;
   for (int d = 0; d < Q; ++d) {
     double v = 0.0;
@@ -180,34 +184,45 @@ __global__ __launch_bounds__(256) void kGather(const S *a, Geom g, int rule, dou
       v = St::scaled ? weightOf(d) * rhoRef + (double)St::load(a[i], (R)inv)
                      : (double)St::load(a[i], (R)inv);
     }
-    out[c.cell * Q + d] = v;
+    out[(c.cell - first) * Q + d] = v;
   }
 }
 
 template <typename S>
 __global__ __launch_bounds__(256) void kScatter(S *a, Geom g, int rule, double rhoRef,
-    float scale, const double *in)
+    float scale, const double *in, uint64_t first, uint64_t end)
 {
 #pragma clang fp contract(off)
   using R  = typename Store<S>::R;
   using St = Store<S>;
-  const CellPos c = cellPos(g);
+  const CellPos c = cellPosIn(g, first, end);
   if (!c.valid)
     return;
   for (int d = 0; d < Q; ++d) {
-    const double v   = in[c.cell * Q + d];
+    const double v   = in[(c.cell - first) * Q + d];
     const uint64_t i = logicalSlot(g, rule, c.x, c.y, c.z, c.cell, d);
     a[i] = St::scaled ? St::store((R)(v - weightOf(d) * rhoRef), (R)scale) : St::store((R)v, (R)scale);
   }
 }
 
-void launchGather(const AuxLaunch &A, double *out, hipStream_t s)
+static void cellRange(const Geom &g, uint64_t &first, uint64_t &count)
 {
-  const uint64_t n      = (uint64_t)A.g.nx * A.g.ny * A.g.nz;
-  const unsigned blocks = (unsigned)((n + 255) / 256);
+  const uint64_t n = (uint64_t)g.nx * g.ny * g.nz;
+  if (first > n)
+    first = n;
+  if (count > n - first)
+    count = n - first;
+}
+
+void launchGather(const AuxLaunch &A, double *out, hipStream_t s, uint64_t first, uint64_t count)
+{
+  cellRange(A.g, first, count);
+  if (count == 0)
+    return;
+  const unsigned blocks = (unsigned)((count + 255) / 256);
 #define LAUNCH(T)                                                                          \
   hipLaunchKernelGGL(kGather<T>, dim3(blocks), dim3(256), 0, s, (const T *)A.pdf, A.g, A.rule,    \
-      A.rhoRef, A.invScale, out)
+      A.rhoRef, A.invScale, out, first, first + count)
   switch (A.storage) {
   case ST_F64: LAUNCH(double); break;
   case ST_F32: LAUNCH(float); break;
@@ -218,13 +233,64 @@ void launchGather(const AuxLaunch &A, double *out, hipStream_t s)
 #undef LAUNCH
 }
 
-void launchScatter(const AuxLaunch &A, const double *in, hipStream_t s)
+void launchScatter(const AuxLaunch &A, const double *in, hipStream_t s, uint64_t first,
+    uint64_t count)
 {
-  const uint64_t n      = (uint64_t)A.g.nx * A.g.ny * A.g.nz;
-  const unsigned blocks = (unsigned)((n + 255) / 256);
+  cellRange(A.g, first, count);
+  if (count == 0)
+    return;
+  const unsigned blocks = (unsigned)((count + 255) / 256);
 #define LAUNCH(T)                                                                          \
   hipLaunchKernelGGL(kScatter<T>, dim3(blocks), dim3(256), 0, s, (T *)A.pdf, A.g, A.rule,         \
-      A.rhoRef, A.scale, in)
+      A.rhoRef, A.scale, in, first, first + count)
+  switch (A.storage) {
+  case ST_F64: LAUNCH(double); break;
+  case ST_F32: LAUNCH(float); break;
+  case ST_F32DEV: LAUNCH(f32dev); break;
+  case ST_F16: LAUNCH(__half); break;
+  default: LAUNCH(__hip_bfloat16); break;
+  }
+#undef LAUNCH
+}
+
+// ---- per-cell density and momentum -----------------------------------------
+// out[(cell - first)*4 + {0,1,2,3}] = rho, jx, jy, jz of the logical PDFs
+// (the incompressible model's velocity is j); 0 on obstacle cells
+template <typename S>
+__global__ __launch_bounds__(256) void kCellMoments(const S *a, Geom g, int rule, double rhoRef,
+    float inv, double *out, uint64_t first, uint64_t end)
+{
+  using R  = typename Store<S>::R;
+  using St = Store<S>;
+  const CellPos c = cellPosIn(g, first, end);
+  if (c.cell >= end)
+    return;
+  double m[4] = { 0.0, 0.0, 0.0, 0.0 };
+  if (c.valid) {
+    for (int d = 0; d < Q; ++d) {
+      const uint64_t i = logicalSlot(g, rule, c.x, c.y, c.z, c.cell, d);
+      const double v   = (double)St::load(a[i], (R)inv);
+      m[0] += v;
+      m[1] += cx(d) * v;
+      m[2] += cy(d) * v;
+      m[3] += cz(d) * v;
+    }
+    m[0] += St::scaled ? rhoRef : 0.0; // sum_d w_d rhoRef
+  }
+  for (int k = 0; k < 4; ++k)
+    out[(c.cell - first) * 4 + k] = m[k];
+}
+
+void launchCellMoments(const AuxLaunch &A, double *out, hipStream_t s, uint64_t first,
+    uint64_t count)
+{
+  cellRange(A.g, first, count);
+  if (count == 0)
+    return;
+  const unsigned blocks = (unsigned)((count + 255) / 256);
+#define LAUNCH(T)                                                                          \
+  hipLaunchKernelGGL(kCellMoments<T>, dim3(blocks), dim3(256), 0, s, (const T *)A.pdf, A.g,      \
+      A.rule, A.rhoRef, A.invScale, out, first, first + count)
   switch (A.storage) {
   case ST_F64: LAUNCH(double); break;
   case ST_F32: LAUNCH(float); break;

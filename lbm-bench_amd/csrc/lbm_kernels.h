@@ -87,8 +87,15 @@ void launchReduceCompare(const double *partials, int n, double *out, hipStream_t
 void launchCopy(const double *src, double *dst, uint64_t n, hipStream_t s);
 // logical PDFs of every cell to / from a float64 [cell][19] buffer
 // (obstacle cells: written as 0 / skipped), converting the storage form
-void launchGather(const AuxLaunch &A, double *out, hipStream_t s);
-void launchScatter(const AuxLaunch &A, const double *in, hipStream_t s);
+// logical [cell][19] float64 of the cells [first, first + count) (x slowest;
+// obstacle cells read as 0 and are not written)
+void launchGather(const AuxLaunch &A, double *out, hipStream_t s, uint64_t first = 0,
+    uint64_t count = ~0ull);
+void launchScatter(const AuxLaunch &A, const double *in, hipStream_t s, uint64_t first = 0,
+    uint64_t count = ~0ull);
+// per-cell rho, jx, jy, jz (float64 [cell][4]) of the cells [first, first + count)
+void launchCellMoments(const AuxLaunch &A, double *out, hipStream_t s, uint64_t first = 0,
+    uint64_t count = ~0ull);
 // solid-mask windows from LatticeDescType.Lattice flags (x fastest):
 // window (row, seg) bit k = (flag at z = (64*seg - 1 + k) mod nz) == obstacle
 void launchBuildMask(const int *lattice, MaskWin *mask, int nx, int ny, int nz, int segs,

@@ -23,6 +23,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "lbm.h"
@@ -760,6 +761,197 @@ void lbmGetNodes(LatticeDescType *ld, KernelDataType *kd, double *out)
   HIP_OK(hipStreamSynchronize(D->stream));
   HIP_OK(hipMemcpy(out, buf, bytes, hipMemcpyDeviceToHost));
   HIP_OK(hipFree(buf));
+}
+
+void lbmCellMoments(LatticeDescType *ld, KernelDataType *kd, double *out)
+{
+  LbmDevice *D = kd->Dev;
+  flushMirror(D);
+  const size_t bytes = sizeof(double) * 4 * D->ncells;
+  double *buf        = nullptr;
+  HIP_OK(hipMalloc(&buf, bytes));
+  launchCellMoments(auxOf(D, currentRule(D, kd)), buf, D->stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(D->stream));
+  HIP_OK(hipMemcpy(out, buf, bytes, hipMemcpyDeviceToHost));
+  HIP_OK(hipFree(buf));
+}
+
+// ---- state files (format in lbm_hip.h) --------------------------------------
+namespace {
+
+const char kStateMagic[8]      = { 'L', 'B', 'M', 'S', 'T', 'A', 'T', '1' };
+const uint32_t kStateVersion   = 1;
+const uint32_t kStateHeader    = 128;
+const uint64_t kFnvBasis       = 0xcbf29ce484222325ull;
+const uint64_t kFnvPrime       = 0x100000001b3ull;
+const size_t kSlabTargetBytes  = 256u << 20; // PDF bytes moved per x-slab
+
+struct StateHeader {
+  char magic[8];
+  uint32_t version, headerBytes;
+  int32_t dims[3], periodic[3], q, iteration;
+  double omega, xforce, magic_;
+  uint64_t flagBytes, pdfBytes, checksum;
+  char kernel[32];
+};
+static_assert(sizeof(StateHeader) == 128, "state header layout");
+
+uint64_t fnvWords(uint64_t h, const void *data, size_t bytes)
+{
+  const uint64_t *w = (const uint64_t *)data;
+  for (size_t i = 0; i < bytes / 8; ++i)
+    h = (h ^ w[i]) * kFnvPrime;
+  return h;
+}
+
+int stateFail(FILE *f, const char *path, const char *what)
+{
+  fprintf(stderr, "ERROR: state file %s: %s\n", path, what);
+  if (f)
+    fclose(f);
+  return -1;
+}
+
+// x-slab length so that one slab holds about kSlabTargetBytes of PDFs
+int slabPlanes(const LbmDevice *D)
+{
+  const size_t plane = (size_t)D->ny * D->nz * Q * sizeof(double);
+  const size_t k     = plane ? kSlabTargetBytes / plane : 1;
+  return (int)std::max<size_t>(1, std::min<size_t>(k, (size_t)D->nx));
+}
+
+std::vector<uint8_t> flagSection(const LatticeDescType *ld, uint64_t bytes)
+{
+  std::vector<uint8_t> v(bytes, 0);
+  const size_t n = (size_t)ld->Dims[0] * ld->Dims[1] * ld->Dims[2];
+  for (size_t i = 0; i < n; ++i)
+    v[i] = (uint8_t)(int8_t)ld->Lattice[i];
+  return v;
+}
+
+} // namespace
+
+int lbmSaveState(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd, const char *path)
+{
+  LbmDevice *D = kd->Dev;
+  flushMirror(D);
+  const size_t ncells = (size_t)D->nx * D->ny * D->nz;
+  StateHeader h;
+  memset(&h, 0, sizeof(h));
+  memcpy(h.magic, kStateMagic, 8);
+  h.version     = kStateVersion;
+  h.headerBytes = kStateHeader;
+  h.dims[0] = D->nx, h.dims[1] = D->ny, h.dims[2] = D->nz;
+  h.periodic[0] = ld->PeriodicX, h.periodic[1] = ld->PeriodicY, h.periodic[2] = ld->PeriodicZ;
+  h.q         = Q;
+  h.iteration = kd->Iteration;
+  if (cd) {
+    const CaseDataType eff = caseDataEffective(cd);
+    h.omega = (double)eff.Omega, h.xforce = (double)eff.XForce, h.magic_ = eff.MagicParam;
+  }
+  h.flagBytes = (ncells + 7) / 8 * 8;
+  h.pdfBytes  = (uint64_t)ncells * Q * sizeof(double);
+  if (kd->Name)
+    strncpy(h.kernel, kd->Name, sizeof(h.kernel) - 1);
+
+  FILE *f = fopen(path, "wb");
+  if (!f)
+    return stateFail(nullptr, path, "cannot open for writing");
+  if (fwrite(&h, sizeof(h), 1, f) != 1)
+    return stateFail(f, path, "write failed");
+  std::vector<uint8_t> flags = flagSection(ld, h.flagBytes);
+  uint64_t sum = fnvWords(kFnvBasis, flags.data(), flags.size());
+  if (fwrite(flags.data(), 1, flags.size(), f) != flags.size())
+    return stateFail(f, path, "write failed");
+
+  const int planes       = slabPlanes(D);
+  const size_t slabCells = (size_t)planes * D->ny * D->nz;
+  double *dbuf           = nullptr;
+  HIP_OK(hipMalloc(&dbuf, slabCells * Q * sizeof(double)));
+  std::vector<double> hbuf(slabCells * Q);
+  const AuxLaunch A = auxOf(D, currentRule(D, kd));
+  int rc            = 0;
+  for (size_t first = 0; first < ncells && rc == 0; first += slabCells) {
+    const size_t count = std::min(slabCells, ncells - first);
+    launchGather(A, dbuf, D->stream, first, count);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(hbuf.data(), dbuf, count * Q * sizeof(double), hipMemcpyDeviceToHost,
+        D->stream));
+    HIP_OK(hipStreamSynchronize(D->stream));
+    sum = fnvWords(sum, hbuf.data(), count * Q * sizeof(double));
+    if (fwrite(hbuf.data(), sizeof(double), count * Q, f) != count * Q)
+      rc = -1;
+  }
+  HIP_OK(hipFree(dbuf));
+  if (rc)
+    return stateFail(f, path, "write failed");
+  h.checksum = sum;
+  if (fseek(f, 0, SEEK_SET) != 0 || fwrite(&h, sizeof(h), 1, f) != 1)
+    return stateFail(f, path, "write failed");
+  if (fclose(f) != 0)
+    return stateFail(nullptr, path, "close failed");
+  return 0;
+}
+
+int lbmLoadState(LatticeDescType *ld, KernelDataType *kd, const char *path)
+{
+  LbmDevice *D = kd->Dev;
+  flushMirror(D);
+  const size_t ncells = (size_t)D->nx * D->ny * D->nz;
+  FILE *f             = fopen(path, "rb");
+  if (!f)
+    return stateFail(nullptr, path, "cannot open for reading");
+  StateHeader h;
+  if (fread(&h, sizeof(h), 1, f) != 1)
+    return stateFail(f, path, "truncated header");
+  if (memcmp(h.magic, kStateMagic, 8) != 0)
+    return stateFail(f, path, "not an LBM state file");
+  if (h.version != kStateVersion || h.headerBytes != kStateHeader || h.q != Q)
+    return stateFail(f, path, "unsupported format version");
+  if (h.dims[0] != D->nx || h.dims[1] != D->ny || h.dims[2] != D->nz)
+    return stateFail(f, path, "lattice dimensions differ");
+  if (h.flagBytes != (ncells + 7) / 8 * 8 || h.pdfBytes != (uint64_t)ncells * Q * sizeof(double))
+    return stateFail(f, path, "section sizes do not match the dimensions");
+  std::vector<uint8_t> flags(h.flagBytes);
+  if (fread(flags.data(), 1, flags.size(), f) != flags.size())
+    return stateFail(f, path, "truncated flag section");
+  for (size_t i = 0; i < ncells; ++i)
+    if ((flags[i] == LAT_CELL_OBSTACLE) != (ld->Lattice[i] == LAT_CELL_OBSTACLE))
+      return stateFail(f, path, "obstacle cells differ from the lattice's");
+  uint64_t sum = fnvWords(kFnvBasis, flags.data(), flags.size());
+
+  const int planes       = slabPlanes(D);
+  const size_t slabCells = (size_t)planes * D->ny * D->nz;
+  double *dbuf           = nullptr;
+  HIP_OK(hipMalloc(&dbuf, slabCells * Q * sizeof(double)));
+  std::vector<double> hbuf(slabCells * Q);
+  const AuxLaunch A = auxOf(D, currentRule(D, kd));
+  int rc            = 0;
+  for (size_t first = 0; first < ncells; first += slabCells) {
+    const size_t count = std::min(slabCells, ncells - first);
+    if (fread(hbuf.data(), sizeof(double), count * Q, f) != count * Q) {
+      rc = -1;
+      break;
+    }
+    sum = fnvWords(sum, hbuf.data(), count * Q * sizeof(double));
+    HIP_OK(hipMemcpyAsync(dbuf, hbuf.data(), count * Q * sizeof(double), hipMemcpyHostToDevice,
+        D->stream));
+    launchScatter(A, dbuf, D->stream, first, count);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(D->stream));
+  }
+  HIP_OK(hipFree(dbuf));
+  D->mirrorValid = false;
+  if (rc)
+    return stateFail(f, path, "truncated PDF section");
+  fclose(f);
+  if (sum != h.checksum) {
+    fprintf(stderr, "ERROR: state file %s: checksum mismatch (the lattice's PDFs are undefined)\n",
+        path);
+    return -1;
+  }
+  return 0;
 }
 
 void lbmSetNodes(LatticeDescType *ld, KernelDataType *kd, const double *in)

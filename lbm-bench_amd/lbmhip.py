@@ -116,6 +116,12 @@ class Library:
         L.kernelSetInitialVelocity.argtypes = [LDP, KDP, CDP]
         L.kernelAddBodyForce.argtypes = [KDP, LDP, CDP]
         L.kernelComputeBoundaryConditions.argtypes = [KDP, LDP, CDP]
+        L.lbmCellMoments.argtypes = [LDP, KDP, dp]
+        L.lbmCellMoments.restype = None
+        L.lbmSaveState.argtypes = [LDP, KDP, CDP, c_char_p]
+        L.lbmSaveState.restype = c_int
+        L.lbmLoadState.argtypes = [LDP, KDP, c_char_p]
+        L.lbmLoadState.restype = c_int
         L.kernelComputeBoundaryConditions.restype = None
         L.lbmAdjointCreate.argtypes = [LDP, CDP, c_int]
         L.lbmAdjointCreate.restype = c_void_p
@@ -209,6 +215,24 @@ class Lattice:
             raise ValueError("set_flags: the obstacle cells cannot change after Init")
         n = self.dims[0] * self.dims[1] * self.dims[2]
         np.ctypeslib.as_array(self.ld.Lattice, shape=(n,))[:] = flags.reshape(-1)
+
+    def cell_moments(self):
+        """float64 (nX, nY, nZ, 4): rho, jx, jy, jz per cell (0 on obstacles)"""
+        out = np.empty(tuple(self.dims) + (4,), dtype=np.float64)
+        self.lib.lib.lbmCellMoments(ctypes.byref(self.ld), self.kd,
+                                    out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out
+
+    def save_state(self, path):
+        """lbmSaveState: flags and logical PDFs to a state file"""
+        if self.lib.lib.lbmSaveState(ctypes.byref(self.ld), self.kd, ctypes.byref(self.cd),
+                                     os.fsencode(path)) != 0:
+            raise OSError(f"lbmSaveState failed for {path}")
+
+    def load_state(self, path):
+        """lbmLoadState: PDFs from a state file of the same lattice"""
+        if self.lib.lib.lbmLoadState(ctypes.byref(self.ld), self.kd, os.fsencode(path)) != 0:
+            raise OSError(f"lbmLoadState failed for {path}")
 
     def boundary_conditions(self, rho_in, rho_out):
         """kernelComputeBoundaryConditions: Zou-He pressure inlet at x = 0 and
@@ -404,3 +428,44 @@ class Adjoint:
 
     def __exit__(self, *exc):
         self.close()
+
+
+STATE_HEADER = np.dtype([("magic", "S8"), ("version", "<u4"), ("header_bytes", "<u4"),
+                         ("dims", "<i4", 3), ("periodic", "<i4", 3), ("q", "<i4"),
+                         ("iteration", "<i4"), ("omega", "<f8"), ("xforce", "<f8"),
+                         ("magic_param", "<f8"), ("flag_bytes", "<u8"), ("pdf_bytes", "<u8"),
+                         ("checksum", "<u8"), ("kernel", "S32")])
+
+
+def read_state(path, verify=True):
+    """A state file of lbmSaveState (format in include/lbm_hip.h) ->
+    (header dict, flags int8 (nZ, nY, nX), logical PDFs float64
+    (nX, nY, nZ, 19), memory-mapped); verify recomputes the checksum"""
+    h = np.fromfile(path, dtype=STATE_HEADER, count=1)[0]
+    if h["magic"] != b"LBMSTAT1" or h["version"] != 1 or h["header_bytes"] != 128:
+        raise ValueError(f"{path}: not a version-1 LBM state file")
+    nx, ny, nz = (int(v) for v in h["dims"])
+    ncells = nx * ny * nz
+    flags = np.memmap(path, dtype=np.int8, mode="r", offset=128, shape=(ncells,))
+    pdfs = np.memmap(path, dtype="<f8", mode="r", offset=128 + int(h["flag_bytes"]),
+                     shape=(nx, ny, nz, int(h["q"])))
+    if verify:
+        words = np.memmap(path, dtype="<u8", mode="r", offset=128,
+                          shape=((int(h["flag_bytes"]) + int(h["pdf_bytes"])) // 8,))
+        if _fnv64_words(words) != int(h["checksum"]):
+            raise ValueError(f"{path}: checksum mismatch")
+    header = {k: (h[k].tolist() if k not in ("magic", "kernel") else h[k].decode())
+              for k in STATE_HEADER.names}
+    return header, flags.reshape(nz, ny, nx), pdfs
+
+
+def _fnv64_words(words, chunk=1 << 22):
+    """FNV-1a over 64-bit words (offset basis 0xcbf29ce484222325, prime
+    0x100000001b3), the state file checksum"""
+    h = 0xCBF29CE484222325
+    prime, mask = 0x100000001B3, (1 << 64) - 1
+    for start in range(0, len(words), chunk):
+        for w in words[start:start + chunk].tolist():
+            h = ((h ^ w) * prime) & mask
+    return h
+
