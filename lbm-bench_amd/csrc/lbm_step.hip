@@ -213,12 +213,13 @@ __global__ __launch_bounds__(256, LBM_MIN_WAVES) void kRowSweep(S *a, S *b,
   };
   S *o = V == V_PUSH ? b : a;
 
-  S carry[Q], pend[Q], first[Q];
+  S carry[Q], pend[Q], first[Q], lcarry[Q];
 #pragma unroll
   for (int d = 0; d < Q; ++d) {
-    carry[d] = S{};
-    pend[d]  = S{};
-    first[d] = S{};
+    carry[d]  = S{};
+    pend[d]   = S{};
+    first[d]  = S{};
+    lcarry[d] = S{};
   }
   bool prevClean = true; // block k - 1 had no solid cell near it
 
@@ -238,17 +239,60 @@ __global__ __launch_bounds__(256, LBM_MIN_WAVES) void kRowSweep(S *a, S *b,
       constexpr bool W = decltype(walls)::value;
       const bool fluid = W ? !__builtin_amdgcn_inverse_ballot_w64(wc.lo) : true;
       R f[Q];
-      if (fluid) {
+      if constexpr (V == V_AA_ODD && sizeof(S) == 8) {
+        // gather from x - c_d.  The z-shifted directions load the block of
+        // the neighbouring row aligned and shift it by one lane.  From
+        // z - 1, lane 0 takes lane 63 of the previous block (element nZ - 1
+        // for block 0).  From z + 1, lane 63 takes element 64k + 64 (0 for
+        // the last block), one uniform load.  Each of these slots is read
+        // and later written only by a cell of this row that has not stored
+        // yet.  All lanes load (a fluid lane may take its value from the lane
+        // of a solid cell); solid cells' results are never stored.  Used for
+        // 8-byte elements only: measured at 512^3, it gains 2 % there, while
+        // the lane shifts cost 2-byte elements 16 %.
 #pragma unroll
         for (int d = 0; d < Q; ++d) {
-          uint64_t i = sl(cell, d);
-          if (V == V_AA_ODD && d != C) {
-            i = sl(cellN(-cx(d), -cy(d), -cz(d)), inv(d));
-            if (W && __builtin_amdgcn_inverse_ballot_w64(bitsOf(-cx(d), -cy(d), -cz(d), k)))
-              i = sl(cell, d);
+          S v;
+          if (cz(d) == 0) {
+            v = getS<NTL>(&a[sl(cellN(-cx(d), -cy(d), 0), inv(d))]);
+          } else {
+            const uint64_t rb = rowStart(-cx(d), -cy(d));
+            if (cz(d) > 0) {
+              const S al   = getS<NTL>(&a[sl(rb + (uint64_t)z, inv(d))]);
+              const S fill = k == 0 ? a[sl(rb + (uint64_t)(nz - 1), inv(d))] : lcarry[d];
+              v            = laneShift(al, fill, true);
+              lcarry[d]    = laneValue(al, 63);
+            } else {
+              const S al   = getS<NTL>(&a[sl(rb + (uint64_t)z, inv(d))]);
+              const int ze = k == K - 1 ? 0 : k * 64 + 64;
+              v            = laneShift(al, a[sl(rb + (uint64_t)ze, inv(d))], false);
+            }
           }
-          f[d] = St::load(getS<NTL>(&a[i]), p.inv);
+          if (W && d != C && __builtin_amdgcn_inverse_ballot_w64(bitsOf(-cx(d), -cy(d), -cz(d), k)))
+            v = getS<NTL>(&a[sl(cell, d)]); // bounce-back: own slot d
+          f[d] = St::load(v, p.inv);
         }
+      } else if constexpr (V == V_AA_ODD) {
+        if (fluid) {
+#pragma unroll
+          for (int d = 0; d < Q; ++d) {
+            uint64_t i = sl(cell, d);
+            if (d != C) {
+              i = sl(cellN(-cx(d), -cy(d), -cz(d)), inv(d));
+              if (W && __builtin_amdgcn_inverse_ballot_w64(bitsOf(-cx(d), -cy(d), -cz(d), k)))
+                i = sl(cell, d);
+            }
+            f[d] = St::load(getS<NTL>(&a[i]), p.inv);
+          }
+        } else {
+#pragma unroll
+          for (int d = 0; d < Q; ++d)
+            f[d] = R(0);
+        }
+      } else if (fluid) {
+#pragma unroll
+        for (int d = 0; d < Q; ++d)
+          f[d] = St::load(getS<NTL>(&a[sl(cell, d)]), p.inv);
       } else {
 #pragma unroll
         for (int d = 0; d < Q; ++d)
