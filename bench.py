@@ -105,6 +105,60 @@ def adjoint_run(lib, lbmhip):
     return out
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_reference_openmp():
+    """LBM-Bench's own pull-soa, which the reference parallelises with
+    OpenMP (collapse(2) over x, y), on up to 16 host threads, 256^3 channel"""
+    exe = os.path.join(ROOT, "oracle", "_ref", "lbmbench-ref")
+    if not os.path.exists(exe):
+        return None
+    threads, iters = min(16, os.cpu_count() or 1), 60
+    cmd = [exe, "-d", "256x256x256", "-g", "channel", "-z", "-k", "pull-soa", "-i", str(iters),
+           "-o", "1.25", "-f", "1e-6"]
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PLACES="cores",
+               OMP_PROC_BIND="close")
+    log(f"cpu reference, {threads} threads: " + " ".join(cmd[1:]))
+    out = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600).stdout
+    m = re.search(r"^P:\s+([0-9.]+) MFLUP/s\s+d: ([0-9.]+) s", out, flags=re.M)
+    if not m:
+        return None
+    return {"value": float(m.group(1)), "unit": "MLUPS", "threads": threads, "kind": "reference",
+            "kernel": "pull-soa", "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "sample": f"LBM-Bench pull-soa (OpenMP), 256^3 channel -z, {iters} steps, "
+                      f"{float(m.group(2)):.1f} s"}
+
+
+def study_variants(lib, lbmhip, n, steps):
+    """the seven variants on an n^3 channel (walls at y, periodic x and z),
+    f = w(1+eps) from seed 0, FP64: MLUPS and GB/s each"""
+    dims = (n, n, n)
+    kw = dict(omega=1.25, xforce=1e-6, body_force=True, magic=3.0 / 16.0)
+    rows = {}
+    for kernel in lbmhip.KERNELS:
+        L = lbmhip.Lattice(lib, kernel, "channel", dims, (0, 0, 1), **kw)
+        L.init("random", 0, 1.0, 1e-3)
+        L.run(20)
+        dur = L.run(steps)
+        ml = L.n_fluid * steps / dur / 1e6
+        gbs = BYTES_PER_LUP["native"] * ml / 1e3
+        rows[kernel] = {"mlups": round(ml, 1), "gb_s": round(gbs, 1),
+                        "frac_of_8tb": round(gbs / HBM_PEAK_GBS, 4)}
+        log(f"{kernel}: {ml:.0f} MLUPS, {gbs:.0f} GB/s")
+        L.close()
+    return {"study": "the seven variants", "workload": f"{n}^3 channel -z, omega 1.25, Lambda "
+            f"3/16, body force 1e-6, f = w(1+eps), {steps} steps, FP64", "variants": rows}
+
+
 def study_precision(lib, lbmhip, n, spheres, total, chunk):
     """The 512^3 spheres workload in FP64, FP32, FP16 and BF16 storage, run
     side by side for `total` steps; after every `chunk` steps the velocity
@@ -198,10 +252,10 @@ def main():
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--study", choices=["precision5000", "adjoint384"],
+    ap.add_argument("--study", choices=["precision5000", "adjoint384", "variants256"],
                     help="instead of the bench line, one JSON line of a longer study: the four "
-                         "storage forms over 5000 steps, or the 384^3 FP16/FP32 adjoint with "
-                         "T = 4096 and two checkpoint levels")
+                         "storage forms over 5000 steps, the 384^3 FP16/FP32 adjoint with "
+                         "T = 4096 and two checkpoint levels, or the seven variants at 256^3")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -217,6 +271,9 @@ def main():
     if args.study == "precision5000":
         print(json.dumps(study_precision(lib, lbmhip, args.size, args.spheres, 5000, 1000)),
               flush=True)
+        return
+    if args.study == "variants256":
+        print(json.dumps(study_variants(lib, lbmhip, 256, 500)), flush=True)
         return
     if args.study == "adjoint384":
         print(json.dumps(study_adjoint(lib, lbmhip, 384, 4096, [64, 8], 5.0)), flush=True)
@@ -280,6 +337,7 @@ def main():
     adjoint = None if args.no_adjoint else adjoint_run(lib, lbmhip)
 
     cpu = None if args.no_cpu else cpu_baseline(20)
+    cpu_omp = None if args.no_cpu else cpu_reference_openmp()
 
     out = {
         "metric": METRIC,
@@ -303,6 +361,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profiles(f"{args.kernel}-fp64-{n}")},
         "cpu_baseline": cpu,
+        "cpu_reference_openmp": cpu_omp,
         "fp16": forms.get("fp16"),
         "fp32": forms.get("fp32"),
         "bf16": forms.get("bf16"),
