@@ -531,7 +531,7 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
     kd->Dims[a]       = ld->Dims[a];
     kd->GlobalDims[a] = ld->Dims[a];
     kd->Offsets[a]    = 0;
-    kd->Blk[a]        = tiled ? 8 : ld->Dims[a];
+    kd->Blk[a]        = tiled ? (a == 2 ? 64 : 2) : ld->Dims[a]; // the tile of lbm_tile.hip
   }
   kd->Iteration = -1;
   kd->Kernel    = kernel;

@@ -1,16 +1,21 @@
 // lbm_tile.hip -- the blocked variants (LBM-Bench src/kernel_blk_pull_soa.c,
 // src/kernel_blk_push_soa.c): the same pull / push step as lbm_step.hip,
-// traversed in 8x8x8 tiles, one workgroup of 512 lanes per tile
-// (lane -> z = t & 7, y = (t >> 3) & 7, x = t >> 6).  Each cell is still
-// independent of the others, so the results equal the unblocked step.
-// Solid bits are looked up per lane; the tile layout does not line up with
-// the 64-wide row windows of the unblocked kernels.
+// traversed in tiles of TZ x TY x TX cells, one workgroup of 256 lanes per
+// tile.  The reference blocks in 8^3 cubes for CPU caches
+// (src/lbm.c:629-650); here a tile is 64 z x 2 y x 2 x, so each wavefront
+// covers 64 consecutive z of one row (lane -> z = t & 63, y = (t >> 6) & 1,
+// x = t >> 7) and its loads and stores are whole 512-byte runs.  An 8^3
+// tile gave each wavefront eight 64-byte runs and ran at half the speed.
+// Each cell is still independent of the others, so the results equal the
+// unblocked step.  Solid bits are looked up per lane.
 #include "lbm_impl.h"
 
 namespace lbm {
 
+constexpr int TZ = 64, TY = 2, TX = 2; // tile edge lengths (TZ * TY * TX = 256 lanes)
+
 template <int V, typename S>
-__global__ __launch_bounds__(512) void kStepTile(S *a, S *b, const MaskWin *__restrict__ mask,
+__global__ __launch_bounds__(256) void kStepTile(S *a, S *b, const MaskWin *__restrict__ mask,
     Params<S> p, int tilesY, int tilesZ)
 {
   using R  = typename Store<S>::R;
@@ -20,9 +25,9 @@ __global__ __launch_bounds__(512) void kStepTile(S *a, S *b, const MaskWin *__re
   const int tz  = bid % tilesZ;
   const int ty  = (bid / tilesZ) % tilesY;
   const int tx  = bid / (tilesZ * tilesY);
-  const int z   = tz * 8 + (t & 7);
-  const int y   = ty * 8 + ((t >> 3) & 7);
-  const int x   = tx * 8 + (t >> 6);
+  const int z   = tz * TZ + (t & (TZ - 1));
+  const int y   = ty * TY + ((t / TZ) & (TY - 1));
+  const int x   = tx * TX + t / (TZ * TY);
   if (x >= p.nx || y >= p.ny || z >= p.nz)
     return;
   if (solidAt(mask, p.segs, p.ny, x, y, z))
@@ -80,10 +85,10 @@ template <int V, typename S>
 static void launchT(const StepLaunch &L, hipStream_t s)
 {
   Params<S> p    = makeParams<S>(L);
-  const int tX   = (p.nx + 7) / 8, tY = (p.ny + 7) / 8, tZ = (p.nz + 7) / 8;
+  const int tX = (p.nx + TX - 1) / TX, tY = (p.ny + TY - 1) / TY, tZ = (p.nz + TZ - 1) / TZ;
   const unsigned blocks = (unsigned)((uint64_t)tX * tY * tZ);
-  hipLaunchKernelGGL((kStepTile<V, S>), dim3(blocks), dim3(512), 0, s, (S *)L.src, (S *)L.dst,
-      L.g.mask, p, tY, tZ);
+  hipLaunchKernelGGL((kStepTile<V, S>), dim3(blocks), dim3(TZ * TY * TX), 0, s, (S *)L.src,
+      (S *)L.dst, L.g.mask, p, tY, tZ);
 }
 
 template <typename S>
