@@ -90,16 +90,18 @@ __device__ __forceinline__ void addForce(R (&f)[Q], R f1, R f2)
 // One wavefront owns 64 consecutive z of one (x, y) row.  The row, the
 // segment and the 3x3 neighbour rows are wavefront-uniform (scalar); each
 // lane holds its z and the wrapped z-1 / z+1.  init() returns false for
-// lanes past the lattice end and for solid cells.
+// lanes past the lattice end and, unless keepSolid, for solid cells; with
+// keepSolid a solid cell's lane stays with fluid == false.
 struct RowCtx {
   const MaskWin *mask;
   int segs, ny, nz;
   int lane, row, seg, z;
   int xs[3], ys[3], zs[3];
   uint64_t cell;
+  bool fluid;
 
   __device__ __forceinline__ bool init(const MaskWin *__restrict__ m, int nx, int ny_, int nz_,
-      int segs_, int remap)
+      int segs_, int remap, bool keepSolid = false)
   {
     mask = m;
     segs = segs_;
@@ -116,7 +118,8 @@ struct RowCtx {
     if (z >= nz)
       return false;
     const int x = row / ny, y = row - x * ny;
-    if (bitAt(mask[(uint64_t)row * segs + seg].lo, lane))
+    fluid       = !bitAt(mask[(uint64_t)row * segs + seg].lo, lane);
+    if (!fluid && !keepSolid)
       return false;
     xs[0] = wrapDec(x, nx);
     xs[1] = x;
