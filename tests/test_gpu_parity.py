@@ -282,3 +282,21 @@ def test_zou_he_matches_oracle(gpu_lib, oracle, kernel):
         run, _ = oracle.run(kernel, dims, lat, per_o, init, steps, 1.25, 1 / 12, 1e-4, True)
         exp = oracle.zou_he(dims, marked, run, 1.01, 0.99)
         assert rel_err(got, exp, fluid) < TOL, (kernel, steps)
+
+
+@pytest.mark.parametrize("kernel", ["aa-soa", "push-soa"])
+def test_thousand_steps_match_oracle(gpu_lib, oracle, kernel):
+    """1000 steps of the row-sweep kernels (two 64-blocks per row, spheres
+    and walls, the force on, Lambda 3/16) against the oracle, every PDF"""
+    dims, steps = (32, 40, 128), 1000
+    lat, per, _ = oracle.geometry("spheres-12", dims, (0, 0, 1), 9)
+    init = oracle.init_random(dims, lat, 5, 1e-2)
+    exp, _ = oracle.run(kernel, dims, lat, per, init, steps, 1.25, 3 / 16, 1e-5, True)
+    fluid = lat.transpose(2, 1, 0) != 0
+    with Lattice(gpu_lib, kernel, "spheres-12", dims, (0, 0, 1), omega=1.25, xforce=1e-5,
+                 body_force=True, magic=3 / 16, seed=9) as L:
+        assert (L.lattice_flags() == lat).all()
+        L.set_nodes(init)
+        L.run(steps)
+        got = L.get_nodes()
+    assert rel_err(got, exp, fluid) < TOL
