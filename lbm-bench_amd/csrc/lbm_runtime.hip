@@ -551,10 +551,13 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
   D->segs    = (D->nz + 63) / 64;
   D->ncells  = (uint64_t)D->nx * D->ny * D->nz;
   D->nfluid  = (uint64_t)ld->nFluid;
-  // SoA planes padded to a multiple of 64 elements plus a 64 KiB stagger, so
-  // the 19 streams of a wavefront do not map to the same HBM channels (the
-  // planes are 1 GiB apart at 512^3); measured best of 0 ... 1 MiB
-  const uint64_t pad = (uint64_t)envInt("LBM_PLANE_PAD", (int)(65536 / storageBytes(storageKind(cd))));
+  // SoA planes padded to a multiple of 64 elements plus a stagger, so the 19
+  // streams of a wavefront need not map to the same HBM channels (the planes
+  // are 1 GiB apart at 512^3 FP64).  Measured over 0 ... 1 MiB with the
+  // current kernels: none is best for 8-byte elements (+3 %), 64 KiB for
+  // 2-byte elements (none loses 4 %), either for 4-byte elements.
+  const size_t ebytes = storageBytes(storageKind(cd));
+  const uint64_t pad  = (uint64_t)envInt("LBM_PLANE_PAD", ebytes == 8 ? 0 : (int)(65536 / ebytes));
   D->plane           = layout == LAYOUT_SOA ? ((D->ncells + 63) / 64) * 64 + pad : D->ncells;
   D->nelem           = layout == LAYOUT_SOA ? D->plane * Q : D->ncells * Q;
   D->narrays         = prop == PROP_AA ? 1 : 2;
@@ -572,7 +575,9 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
   if (D->storage == kNativeStorage)
     D->rhoRef = 0.0;
   D->gsteps  = envInt("LBM_GRAPH_STEPS", 16) & ~1;
-  D->remap   = envInt("LBM_XCD_REMAP", 1);
+  // XCD-aware block order: measured at 512^3 with the current kernels it
+  // loses 1-3 %, so it is off unless asked for
+  D->remap   = envInt("LBM_XCD_REMAP", 0);
   // measured at 512^3: 8-byte elements stream best with non-temporal loads
   // and stores; 4- and 2-byte elements with non-temporal stores only
   D->nt      = envInt("LBM_NT", D->esize == 8 ? 3 : 1);
