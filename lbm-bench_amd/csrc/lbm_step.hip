@@ -1,15 +1,21 @@
-// lbm_step.hip -- the stream-and-collide step, one thread per cell along
-// the unit-stride z axis.
+// lbm_step.hip -- the stream-and-collide step of the row-ordered variants
+// (LBM-Bench src/kernel_{pull,push,aa}_soa.c, src/kernel_{pull,push}_aos.c).
 //
-// Mapping: a wavefront owns 64 consecutive z of one (x, y) row
-// (wave w -> row w / segs, segment w % segs); 4 wavefronts per workgroup.
+// Three kernels share the slot rules below:
+//   kStepRow   SoA, one wavefront per 64 consecutive z of one (x, y) row
+//              (wave w -> row w / segs, segment w % segs), 4 per workgroup;
+//              pull, push, AA even and AA odd.
+//   kRowSweep  SoA push and AA odd with nZ % 64 == 0: one wavefront walks a
+//              whole row so that its z-shifted stores (and, for 8-byte
+//              elements, loads) are line-aligned through DPP lane shifts.
+//   kStepAoS   AoS push and pull: the side that touches the wavefront's own
+//              64 records goes through LDS.
 // Row, segment and the nine neighbouring rows are wavefront-uniform, so the
-// nine mask windows of the 3x3 row neighbourhood come in as scalar loads
-// and each lane picks its z-1 / z / z+1 solid bits by shifting them.
+// mask windows of the 3x3 row neighbourhood come in as scalar loads and
+// each lane picks its z-1 / z / z+1 solid bits by shifting them.
 //
-// Variants (LBM-Bench src/kernel_*.c), all with half-way bounce-back folded
-// into the addressing through the mask instead of the reference's
-// post-pass copy lists (src/lbm.c:465-568):
+// Half-way bounce-back is folded into the addressing through the mask
+// instead of the reference's post-pass copy lists (src/lbm.c:465-568):
 //   pull    f_d = solid(x-c_d) ? src[x, inv d] : src[x-c_d, d]; dst[x, d] = f*_d
 //   push    f_d = src[x, d]; solid(x+c_d) ? dst[x, inv d] : dst[x+c_d, d] = f*_d
 //   AA even f_d = a[x, d];   a[x, inv d] = f*_d            (in place)
@@ -17,8 +23,9 @@
 //           solid(x+c_d) ? a[x, inv d] : a[x+c_d, d] = f*_d   (in place)
 // In the AA odd step the slot (y, k) is read and written only by cell
 // y - c_k, or, when that cell is solid, by nobody else than y itself
-// through the bounce-back branch, so cells never race.  Solid cells are
-// skipped entirely.
+// through the bounce-back branch, so cells never race.  No step reads the
+// slots of a solid cell; kernels that store to their own cell write zeros
+// there so that their store lines are whole.
 #include "lbm_impl.h"
 
 namespace lbm {
@@ -28,7 +35,7 @@ namespace lbm {
 #define LBM_MIN_WAVES 1
 #endif
 
-template <int V, bool AOS, int NT, typename S>
+template <int V, int NT, typename S>
 __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const MaskWin *__restrict__ mask,
     Params<S> p, int remap)
 {
@@ -60,20 +67,20 @@ __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const
     for (int d = 0; d < Q; ++d) {
       uint64_t i;
       if (d == C) {
-        i = slot<AOS>(cell, C, p.plane);
+        i = slot<false>(cell, C, p.plane);
       } else {
         const bool bb = solidOf(-cx(d), -cy(d), -cz(d));
         if (V == V_PULL)
-          i = bb ? slot<AOS>(cell, inv(d), p.plane) : slot<AOS>(cellOf(-cx(d), -cy(d), -cz(d)), d, p.plane);
+          i = bb ? slot<false>(cell, inv(d), p.plane) : slot<false>(cellOf(-cx(d), -cy(d), -cz(d)), d, p.plane);
         else
-          i = bb ? slot<AOS>(cell, d, p.plane) : slot<AOS>(cellOf(-cx(d), -cy(d), -cz(d)), inv(d), p.plane);
+          i = bb ? slot<false>(cell, d, p.plane) : slot<false>(cellOf(-cx(d), -cy(d), -cz(d)), inv(d), p.plane);
       }
       f[d] = St::load(getS<NTL>(&a[i]), p.inv);
     }
   } else {
 #pragma unroll
     for (int d = 0; d < Q; ++d)
-      f[d] = St::load(getS<NTL>(&a[slot<AOS>(cell, d, p.plane)]), p.inv);
+      f[d] = St::load(getS<NTL>(&a[slot<false>(cell, d, p.plane)]), p.inv);
   }
 
   if (p.addForce)
@@ -85,13 +92,13 @@ __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const
   if constexpr (V == V_PULL) {
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
-      const uint64_t i = slot<AOS>(cell, d, p.plane);
+      const uint64_t i = slot<false>(cell, d, p.plane);
       putS<NTS>(&b[i], St::storeStep(f[d], p.scale, ditherDir(dz, d)));
     }
   } else if constexpr (V == V_AA_EVEN) {
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
-      const uint64_t i = slot<AOS>(cell, inv(d), p.plane);
+      const uint64_t i = slot<false>(cell, inv(d), p.plane);
       putS<NTS>(&a[i], St::storeStep(f[d], p.scale, ditherDir(dz, d)));
     }
   } else {
@@ -100,10 +107,10 @@ __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const
     for (int d = 0; d < Q; ++d) {
       uint64_t i;
       if (d == C) {
-        i = slot<AOS>(cell, C, p.plane);
+        i = slot<false>(cell, C, p.plane);
       } else {
         const bool bb = solidOf(cx(d), cy(d), cz(d));
-        i = bb ? slot<AOS>(cell, inv(d), p.plane) : slot<AOS>(cellOf(cx(d), cy(d), cz(d)), d, p.plane);
+        i = bb ? slot<false>(cell, inv(d), p.plane) : slot<false>(cellOf(cx(d), cy(d), cz(d)), d, p.plane);
       }
       putS<NTS>(&o[i], St::storeStep(f[d], p.scale, ditherDir(dz, d)));
     }
@@ -527,17 +534,19 @@ static void launchRowSweep(const StepLaunch &L, hipStream_t s)
 template <int V, bool AOS, int NT, typename S>
 static void launchT(const StepLaunch &L, hipStream_t s)
 {
-  if constexpr (!AOS && (V == V_PUSH || V == V_AA_ODD)) {
-    if (L.rowSweep && L.g.nz % 64 == 0 && L.g.plane < (1ull << 32))
-      return launchRowSweep<V, NT, S>(L, s);
+  if constexpr (AOS) {
+    launchAoS<V, NT, S>(L, s);
+  } else {
+    if constexpr (V == V_PUSH || V == V_AA_ODD) {
+      if (L.rowSweep && L.g.nz % 64 == 0 && L.g.plane < (1ull << 32))
+        return launchRowSweep<V, NT, S>(L, s);
+    }
+    Params<S> p       = makeParams<S>(L);
+    const int wpb     = L.wavesPerBlock == 8 ? 8 : 4;
+    const unsigned blocks = rowBlocks(L.g, wpb);
+    hipLaunchKernelGGL((kStepRow<V, NT, S>), dim3(blocks), dim3(64 * wpb), 0, s, (S *)L.src,
+        (S *)L.dst, L.g.mask, p, L.remap);
   }
-  if constexpr (AOS && (V == V_PUSH || V == V_PULL))
-    return launchAoS<V, NT, S>(L, s);
-  Params<S> p       = makeParams<S>(L);
-  const int wpb     = L.wavesPerBlock == 8 ? 8 : 4;
-  const unsigned blocks = rowBlocks(L.g, wpb);
-  hipLaunchKernelGGL((kStepRow<V, AOS, NT, S>), dim3(blocks), dim3(64 * wpb), 0, s, (S *)L.src,
-      (S *)L.dst, L.g.mask, p, L.remap);
 }
 
 template <int V, bool AOS, typename S>
