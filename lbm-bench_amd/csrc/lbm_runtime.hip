@@ -333,6 +333,10 @@ static void launchStep(LbmDevice *D, StepLaunch L, int variant, int srcIdx, int 
 {
   L.variant  = variant;
   L.addForce = addForce;
+  // measured at 512^3: the XCD-aware order gains 5 % for the AA even step
+  // of 4-byte elements and loses 1-3 % elsewhere (row sweep, FP16, FP64)
+  if (D->remap < 0)
+    L.remap = variant == V_AA_EVEN && D->esize == 4 ? 1 : 0;
   L.seqBase  = D->dseq;
   L.seq      = node >= 0 ? (uint32_t)node : eagerSeq(D);
   if (D->prop == PROP_AA) {
@@ -575,9 +579,9 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
   if (D->storage == kNativeStorage)
     D->rhoRef = 0.0;
   D->gsteps  = envInt("LBM_GRAPH_STEPS", 16) & ~1;
-  // XCD-aware block order: measured at 512^3 with the current kernels it
-  // loses 1-3 %, so it is off unless asked for
-  D->remap   = envInt("LBM_XCD_REMAP", 0);
+  // XCD-aware block order (LBM_XCD_REMAP 0 / 1; unset: per launch, see
+  // launchStep)
+  D->remap   = envInt("LBM_XCD_REMAP", -1);
   // measured at 512^3: 8-byte elements stream best with non-temporal loads
   // and stores; 4- and 2-byte elements with non-temporal stores only
   D->nt      = envInt("LBM_NT", D->esize == 8 ? 3 : 1);
