@@ -159,6 +159,32 @@ def study_variants(lib, lbmhip, n, steps):
             f"3/16, body force 1e-6, f = w(1+eps), {steps} steps, FP64", "variants": rows}
 
 
+def study_ceiling(lib, n):
+    """the step's loads and stores without arithmetic (lbmPatternBandwidth) on
+    n^3 cells: GB/s by plane count, in place or to a second array, and by
+    element size; the copy kernel beside them"""
+    cells = n ** 3
+    rows = []
+    for planes in (1, 2, 4, 8, 19):
+        for in_place in (True, False):
+            rows.append((planes, 8, in_place))
+    rows += [(19, 4, True), (19, 2, True)]
+    out = []
+    for planes, eb, in_place in rows:
+        gbs = lib.pattern_bandwidth(planes, eb, in_place, cells, 6)
+        ms = 2.0 * planes * cells * eb / (gbs * 1e9) * 1e3
+        out.append({"planes": planes, "element_bytes": eb, "in_place": in_place,
+                    "gb_s": round(gbs, 1), "frac_of_8tb": round(gbs / HBM_PEAK_GBS, 4),
+                    "ms_per_launch": round(ms, 3)})
+        log(f"{planes:2d} planes x {eb} B, {'in place' if in_place else 'two arrays'}: "
+            f"{gbs:.0f} GB/s, {ms:.3f} ms")
+    copy_gbs = lib.copy_bandwidth(4 << 30, 10)
+    log(f"copy (4 GiB): {copy_gbs:.0f} GB/s")
+    return {"study": "access-pattern ceiling", "workload": f"{n}^3 cells per plane, non-temporal "
+            "loads and stores, one element per lane, 512-lane blocks, 6 launches each",
+            "patterns": out, "copy_bandwidth_gbs": round(copy_gbs, 1)}
+
+
 def study_precision(lib, lbmhip, n, spheres, total, chunk):
     """The 512^3 spheres workload in FP64, FP32, FP16 and BF16 storage, run
     side by side for `total` steps; after every `chunk` steps the velocity
@@ -252,10 +278,11 @@ def main():
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--study", choices=["precision5000", "adjoint384", "variants256"],
+    ap.add_argument("--study", choices=["precision5000", "adjoint384", "variants256", "ceiling"],
                     help="instead of the bench line, one JSON line of a longer study: the four "
                          "storage forms over 5000 steps, the 384^3 FP16/FP32 adjoint with "
-                         "T = 4096 and two checkpoint levels, or the seven variants at 256^3")
+                         "T = 4096 and two checkpoint levels, the seven variants at 256^3, or "
+                         "the step's memory traffic without arithmetic at --size^3")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -275,6 +302,9 @@ def main():
     if args.study == "variants256":
         print(json.dumps(study_variants(lib, lbmhip, 256, 500)), flush=True)
         return
+    if args.study == "ceiling":
+        print(json.dumps(study_ceiling(lib, args.size)), flush=True)
+        return
     if args.study == "adjoint384":
         print(json.dumps(study_adjoint(lib, lbmhip, 384, 4096, [64, 8], 5.0)), flush=True)
         return
@@ -286,6 +316,9 @@ def main():
 
     copy_gbs = lib.copy_bandwidth(4 << 30, 10)
     log(f"device copy bandwidth {copy_gbs:.0f} GB/s")
+    # the AA step's loads and stores without arithmetic, on this lattice's size
+    pattern_gbs = lib.pattern_bandwidth(19, 8, True, n ** 3, 6)
+    log(f"19-plane in-place pattern bandwidth {pattern_gbs:.0f} GB/s")
 
     result = {}
     L64 = lbmhip.Lattice(lib, args.kernel, geo, dims, (0, 0, 1), **kw)
@@ -369,6 +402,7 @@ def main():
         "adjoint": adjoint,
         "mass_drift_rel": (m1[0] - m0[0]) / m0[0],
         "copy_bandwidth_gbs": round(copy_gbs, 1),
+        "pattern_bandwidth_gbs": round(pattern_gbs, 1),
         "device": dev,
     }
     print(json.dumps(out), flush=True)

@@ -92,6 +92,15 @@ size_t lbmDeviceBytes(KernelDataType *kd);
  * achievable-bandwidth line beside the roofline. */
 double lbmCopyBandwidth(size_t bytes, int reps);
 
+/* The memory traffic of one SoA step without its arithmetic, as GB/s (read
+ * + write bytes / second, hipEvents over `reps` launches): every lane loads
+ * element `cell` of `planes` planes of `cells` elements and stores it back,
+ * to the reversed planes of the same array (inPlace != 0, the AA step's
+ * traffic) or to a second array (push and pull).  planes 1, 2, 4, 8 or 19;
+ * elementBytes 2, 4 or 8; returns -1 after a message on stderr otherwise.
+ * The ceiling the step kernels are measured against, beside the copy. */
+double lbmPatternBandwidth(int planes, int elementBytes, int inPlace, size_t cells, int reps);
+
 /* hipDeviceSynchronize; 0 on success */
 int lbmDeviceSynchronize(void);
 

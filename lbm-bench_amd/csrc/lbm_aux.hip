@@ -493,6 +493,55 @@ void launchCopy(const double *src, double *dst, uint64_t n, hipStream_t s)
   hipLaunchKernelGGL(kCopy, dim3(blocks), dim3(256), 0, s, (const v2d *)src, (v2d *)dst, n2);
 }
 
+// ---- access-pattern ceiling --------------------------------------------------
+// The memory traffic of one SoA step without its arithmetic: lane `cell`
+// loads element `cell` of P planes (non-temporal) and stores each value
+// plus one, to plane P-1-q of the same array (in place, the AA step's
+// traffic) or to plane q of a second array (push and pull).
+template <typename T, int P>
+__global__ __launch_bounds__(512) void kPattern(const T *src, T *dst, uint64_t plane, uint64_t cells,
+    int inPlace)
+{
+  const uint64_t c = (uint64_t)blockIdx.x * 512u + threadIdx.x;
+  if (c >= cells)
+    return;
+  T v[P];
+#pragma unroll
+  for (int q = 0; q < P; ++q)
+    v[q] = __builtin_nontemporal_load(&src[c + (uint64_t)q * plane]);
+#pragma unroll
+  for (int q = 0; q < P; ++q)
+    __builtin_nontemporal_store((T)(v[q] + 1), &dst[c + (uint64_t)(inPlace ? P - 1 - q : q) * plane]);
+}
+
+template <typename T>
+static bool launchPatternT(int planes, const void *src, void *dst, uint64_t cells, bool inPlace,
+    hipStream_t s)
+{
+  const unsigned blocks = (unsigned)((cells + 511) / 512);
+  const T *a            = (const T *)src;
+  T *b                  = (T *)dst;
+  switch (planes) {
+  case 1: hipLaunchKernelGGL((kPattern<T, 1>), dim3(blocks), dim3(512), 0, s, a, b, cells, cells, (int)inPlace); return true;
+  case 2: hipLaunchKernelGGL((kPattern<T, 2>), dim3(blocks), dim3(512), 0, s, a, b, cells, cells, (int)inPlace); return true;
+  case 4: hipLaunchKernelGGL((kPattern<T, 4>), dim3(blocks), dim3(512), 0, s, a, b, cells, cells, (int)inPlace); return true;
+  case 8: hipLaunchKernelGGL((kPattern<T, 8>), dim3(blocks), dim3(512), 0, s, a, b, cells, cells, (int)inPlace); return true;
+  case 19: hipLaunchKernelGGL((kPattern<T, 19>), dim3(blocks), dim3(512), 0, s, a, b, cells, cells, (int)inPlace); return true;
+  default: return false;
+  }
+}
+
+bool launchPattern(int planes, int elementBytes, const void *src, void *dst, uint64_t cells,
+    bool inPlace, hipStream_t s)
+{
+  switch (elementBytes) {
+  case 2: return launchPatternT<uint16_t>(planes, src, dst, cells, inPlace, s);
+  case 4: return launchPatternT<uint32_t>(planes, src, dst, cells, inPlace, s);
+  case 8: return launchPatternT<uint64_t>(planes, src, dst, cells, inPlace, s);
+  default: return false;
+  }
+}
+
 } // namespace lbm
 
 namespace lbm {

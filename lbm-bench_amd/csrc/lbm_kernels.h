@@ -85,6 +85,11 @@ int launchCompareFields(const double *fa, const double *fb, uint64_t ncells, dou
 void launchReduceCompare(const double *partials, int n, double *out, hipStream_t s);
 // plain device copy, for the measured bandwidth line and counter scaling
 void launchCopy(const double *src, double *dst, uint64_t n, hipStream_t s);
+// the step's memory traffic without arithmetic over `planes` planes of
+// `cells` elements of `elementBytes` bytes (planes 1, 2, 4, 8 or 19;
+// elementBytes 2, 4 or 8; false for any other); src == dst when in place
+bool launchPattern(int planes, int elementBytes, const void *src, void *dst, uint64_t cells,
+    bool inPlace, hipStream_t s);
 // logical PDFs of every cell to / from a float64 [cell][19] buffer
 // (obstacle cells: written as 0 / skipped), converting the storage form
 // logical [cell][19] float64 of the cells [first, first + count) (x slowest;

@@ -786,6 +786,8 @@ void lbmCellMoments(LatticeDescType *ld, KernelDataType *kd, double *out)
   HIP_OK(hipFree(buf));
 }
 
+} // extern "C"
+
 // ---- state files (format in lbm_hip.h) --------------------------------------
 namespace {
 
@@ -840,6 +842,8 @@ std::vector<uint8_t> flagSection(const LatticeDescType *ld, uint64_t bytes)
 }
 
 } // namespace
+
+extern "C" {
 
 int lbmSaveState(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd, const char *path)
 {
@@ -1032,6 +1036,49 @@ double lbmCopyBandwidth(size_t bytes, int reps)
   HIP_OK(hipEventDestroy(e1));
   HIP_OK(hipStreamDestroy(s));
   return 2.0 * (double)(n / 2 * 2) * sizeof(double) * reps / ((double)ms * 1e-3) / 1e9;
+}
+
+double lbmPatternBandwidth(int planes, int elementBytes, int inPlace, size_t cells, int reps)
+{
+  const bool okPlanes = planes == 1 || planes == 2 || planes == 4 || planes == 8 || planes == 19;
+  const bool okBytes  = elementBytes == 2 || elementBytes == 4 || elementBytes == 8;
+  if (!okPlanes || !okBytes || cells == 0 || reps < 1) {
+    fprintf(stderr, "lbmPatternBandwidth: planes must be 1, 2, 4, 8 or 19, elementBytes 2, 4 "
+                    "or 8, cells and reps positive (got %d, %d, %zu, %d)\n",
+        planes, elementBytes, cells, reps);
+    return -1.0;
+  }
+  const size_t bytes = (size_t)planes * cells * (size_t)elementBytes;
+  void *a = nullptr, *b = nullptr;
+  hipStream_t s;
+  hipEvent_t e0, e1;
+  HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  HIP_OK(hipMalloc(&a, bytes));
+  HIP_OK(hipMemsetAsync(a, 0, bytes, s));
+  if (inPlace) {
+    b = a;
+  } else {
+    HIP_OK(hipMalloc(&b, bytes));
+    HIP_OK(hipMemsetAsync(b, 0, bytes, s));
+  }
+  launchPattern(planes, elementBytes, a, b, cells, inPlace != 0, s);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r)
+    launchPattern(planes, elementBytes, a, b, cells, inPlace != 0, s);
+  HIP_OK(hipEventRecord(e1, s));
+  HIP_OK(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+  HIP_OK(hipFree(a));
+  if (!inPlace)
+    HIP_OK(hipFree(b));
+  HIP_OK(hipEventDestroy(e0));
+  HIP_OK(hipEventDestroy(e1));
+  HIP_OK(hipStreamDestroy(s));
+  return 2.0 * (double)bytes * reps / ((double)ms * 1e-3) / 1e9;
 }
 
 int lbmDeviceSynchronize(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : 1; }

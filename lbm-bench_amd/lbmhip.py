@@ -106,6 +106,8 @@ class Library:
         L.lbmDeviceBytes.restype = ctypes.c_size_t
         L.lbmCopyBandwidth.argtypes = [ctypes.c_size_t, c_int]
         L.lbmCopyBandwidth.restype = c_double
+        L.lbmPatternBandwidth.argtypes = [c_int, c_int, c_int, ctypes.c_size_t, c_int]
+        L.lbmPatternBandwidth.restype = c_double
         L.lbmDeviceSynchronize.restype = c_int
         L.lbmDeviceInfo.argtypes = [c_char_p, c_int, ctypes.POINTER(c_int),
                                     ctypes.POINTER(ctypes.c_size_t)]
@@ -155,6 +157,13 @@ class Library:
 
     def copy_bandwidth(self, nbytes, reps):
         return self.lib.lbmCopyBandwidth(nbytes, reps)
+
+    def pattern_bandwidth(self, planes, element_bytes, in_place, cells, reps):
+        """GB/s of the step's loads and stores without arithmetic"""
+        gbs = self.lib.lbmPatternBandwidth(planes, element_bytes, int(bool(in_place)), cells, reps)
+        if gbs < 0:
+            raise ValueError("pattern_bandwidth: unsupported planes / element size")
+        return gbs
 
     def synchronize(self):
         if self.lib.lbmDeviceSynchronize() != 0:

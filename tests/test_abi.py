@@ -46,6 +46,15 @@ def test_library_exports_every_declared_symbol(precision):
     assert not missing, missing
 
 
+@pytest.mark.parametrize("planes,element_bytes,cells,reps",
+                         [(3, 8, 1024, 1), (19, 3, 1024, 1), (19, 8, 0, 1), (19, 8, 1024, 0)])
+def test_pattern_bandwidth_rejects_unsupported_shapes(planes, element_bytes, cells, reps):
+    """argument checks run before any device call"""
+    lib = lbmhip.Library("dp")
+    with pytest.raises(ValueError):
+        lib.pattern_bandwidth(planes, element_bytes, True, cells, reps)
+
+
 def test_registry_names_and_order():
     lib = lbmhip.Library("dp")
     assert lib.kernel_names() == list(lbmhip.KERNELS)

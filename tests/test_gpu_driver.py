@@ -71,3 +71,14 @@ def test_reference_driver_runs_on_the_gpu(precision):
         norm = float(re.search(r"deviation from analytical solution: (\S+)", r.stdout).group(1))
         tol = 1e-6 * ver[kernel] if precision == "dp" else 2 * spread
         assert abs(norm - ver[kernel]) <= tol, (kernel, norm, ver[kernel])
+
+
+@pytest.mark.parametrize("planes,element_bytes,in_place", [(19, 8, True), (19, 8, False),
+                                                            (19, 4, True), (19, 2, True),
+                                                            (1, 8, False)])
+def test_pattern_bandwidth_measures(planes, element_bytes, in_place):
+    """the ceiling kernel runs and reports a rate between 1 GB/s and the 8 TB/s
+    the part is quoted at (small lattice: launch-bound, so no tighter bound)"""
+    lib = lbmhip.Library("dp")
+    gbs = lib.pattern_bandwidth(planes, element_bytes, in_place, 1 << 22, 3)
+    assert 1.0 < gbs < 8000.0
