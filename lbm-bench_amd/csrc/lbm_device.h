@@ -56,10 +56,13 @@ __host__ __device__ constexpr int wclass(int d)
 }
 
 // TRT constants, formed on the host in the order of
-// src/kernel_pull_soa.c:18-28 with Lambda as a parameter.
+// src/kernel_pull_soa.c:18-28 with Lambda as a parameter.  d15 is the 3/2
+// of D = rho - 3/2 |j|^2.  A collision on values scaled by s (the storage
+// scale, a power of two) uses w1nh / s, w2nh / s and d15 / s: every term is
+// then s times the unscaled one, rounding included.
 template <typename R>
 struct Trt {
-  R we, wo, w0, w1, w2, w1x3, w2x3, w1nh, w2nh;
+  R we, wo, w0, w1, w2, w1x3, w2x3, w1nh, w2nh, d15;
 };
 
 // TRT collision in place (src/kernel_pull_soa.c:118-198): f holds the 19
@@ -73,7 +76,7 @@ __device__ __forceinline__ void collide(R (&f)[Q], const Trt<R> &p)
   const R uz = f[T] + f[TE] + f[TW] + f[TN] + f[TS] - f[B] - f[BE] - f[BW] - f[BN] - f[BS];
   const R rho = f[C] + f[N] + f[E] + f[S] + f[W] + f[NE] + f[SE] + f[SW] + f[NW] + f[T] +
                 f[TN] + f[TE] + f[TS] + f[TW] + f[B] + f[BN] + f[BE] + f[BS] + f[BW];
-  const R D = rho - (ux * ux + uy * uy + uz * uz) * R(3.0) / R(2.0);
+  const R D = rho - (ux * ux + uy * uy + uz * uz) * p.d15;
 
   f[C] = f[C] - p.we * (f[C] - p.w0 * D);
 
@@ -103,8 +106,9 @@ __device__ __forceinline__ void collide(R (&f)[Q], const Trt<R> &p)
 // to the deviation directly: the incompressible equilibrium is linear in
 // rho, so the w*RhoRef part drops out of every term.
 //
-// store() rounds to nearest even.  storeStep() is what the step kernels
-// call: for FP16 and BF16 it rounds with gfx950's stochastic-rounding
+// store() rounds to nearest even.  raw() and rawStep() convert without the
+// scale: the step kernels collide on scaled values (constants in Params).
+// storeStep() scales and rounds like rawStep(): for FP16 and BF16 it rounds with gfx950's stochastic-rounding
 // converts (v_cvt_sr_f16_f32, v_cvt_sr_bf16_f32).  These add bits 16..31
 // of a 32-bit seed below the last kept bit and truncate, so a seed that is
 // uniform over those bits rounds up with probability (dropped part /
@@ -155,6 +159,8 @@ struct Store<double> {
   __device__ static R load(double v, R) { return v; }
   __device__ static double store(R v, R) { return v; }
   __device__ static double storeStep(R v, R, Dither) { return v; }
+  __device__ static R raw(double v) { return v; }
+  __device__ static double rawStep(R v, Dither) { return v; }
 };
 template <>
 struct Store<float> {
@@ -163,6 +169,8 @@ struct Store<float> {
   __device__ static R load(float v, R) { return v; }
   __device__ static float store(R v, R) { return v; }
   __device__ static float storeStep(R v, R, Dither) { return v; }
+  __device__ static R raw(float v) { return v; }
+  __device__ static float rawStep(R v, Dither) { return v; }
 };
 // FP32 shifted-deviation storage: same bytes as float, scaled
 struct f32dev {
@@ -175,6 +183,8 @@ struct Store<f32dev> {
   __device__ static R load(f32dev v, R inv) { return v.v * inv; }
   __device__ static f32dev store(R v, R sc) { return f32dev{ v * sc }; }
   __device__ static f32dev storeStep(R v, R sc, Dither) { return f32dev{ v * sc }; }
+  __device__ static R raw(f32dev v) { return v.v; }
+  __device__ static f32dev rawStep(R v, Dither) { return f32dev{ v }; }
 };
 template <>
 struct Store<__half> {
@@ -182,9 +192,10 @@ struct Store<__half> {
   static constexpr bool scaled = true;
   __device__ static R load(__half v, R inv) { return __half2float(v) * inv; }
   __device__ static __half store(R v, R sc) { return __float2half_rn(v * sc); }
-  __device__ static __half storeStep(R v, R sc, Dither z)
+  __device__ static __half storeStep(R v, R sc, Dither z) { return rawStep(v * sc, z); }
+  __device__ static R raw(__half v) { return __half2float(v); }
+  __device__ static __half rawStep(R x, Dither z)
   {
-    const float x       = v * sc;
     const uint32_t seed = (__float_as_uint(x) & z.mask) ^ z.salt;
     const half2v r      = __builtin_amdgcn_cvt_sr_f16_f32(half2v{ 0, 0 }, x, seed, false);
     return __half(__half_raw{ r[0] });
@@ -196,9 +207,10 @@ struct Store<__hip_bfloat16> {
   static constexpr bool scaled = true;
   __device__ static R load(__hip_bfloat16 v, R inv) { return __bfloat162float(v) * inv; }
   __device__ static __hip_bfloat16 store(R v, R sc) { return __float2bfloat16(v * sc); }
-  __device__ static __hip_bfloat16 storeStep(R v, R sc, Dither z)
+  __device__ static __hip_bfloat16 storeStep(R v, R sc, Dither z) { return rawStep(v * sc, z); }
+  __device__ static R raw(__hip_bfloat16 v) { return __bfloat162float(v); }
+  __device__ static __hip_bfloat16 rawStep(R x, Dither z)
   {
-    const float x       = v * sc;
     const uint32_t seed = (__float_as_uint(x) & z.mask) ^ z.salt;
     const bf162v r      = __builtin_amdgcn_cvt_sr_bf16_f32(bf162v{ 0, 0 }, x, seed, false);
     return __hip_bfloat16(__hip_bfloat16_raw{ __builtin_bit_cast(unsigned short, r[0]) });

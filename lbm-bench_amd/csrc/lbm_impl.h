@@ -26,12 +26,12 @@ inline Trt<R> toTrt(const double *d, const float *f);
 template <>
 inline Trt<double> toTrt<double>(const double *d, const float *)
 {
-  return Trt<double>{ d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8] };
+  return Trt<double>{ d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8], 1.5 };
 }
 template <>
 inline Trt<float> toTrt<float>(const double *, const float *f)
 {
-  return Trt<float>{ f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8] };
+  return Trt<float>{ f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8], 1.5f };
 }
 
 template <typename S>
@@ -54,6 +54,15 @@ inline Params<S> makeParams(const StepLaunch &L)
   p.dither   = L.dither;
   p.seqBase  = L.seqBase;
   p.seq      = L.seq;
+  if (Store<S>::scaled) {
+    // the step kernels collide on the stored (scaled) values: the terms
+    // quadratic in j and the force take the scale in (exact: a power of two)
+    p.trt.w1nh *= p.inv;
+    p.trt.w2nh *= p.inv;
+    p.trt.d15 *= p.inv;
+    p.f1 *= p.scale;
+    p.f2 *= p.scale;
+  }
   return p;
 }
 
@@ -69,6 +78,23 @@ template <bool AOS>
 __device__ __forceinline__ uint64_t slot(uint64_t cell, int d, uint64_t plane)
 {
   return AOS ? cell * Q + (uint64_t)d : (uint64_t)d * plane + cell;
+}
+
+// A pointer the compiler must take as it is: the slot offset d * plane
+// stays in the (wavefront-uniform, scalar) base instead of being folded
+// into every lane's address as a 64-bit vector multiply-add, so a lane's
+// access costs one 64-bit add.  Not volatile: equal bases are computed once.
+template <typename T>
+__device__ __forceinline__ T *scalarBase(T *p)
+{
+  asm("" : "+s"(p));
+  return p;
+}
+// element `cell` of SoA slot d
+template <typename T>
+__device__ __forceinline__ T *slotPtr(T *a, int d, uint64_t plane, uint64_t cell)
+{
+  return scalarBase(a + (uint64_t)d * plane) + cell;
 }
 
 // body force on the 10 directions with c_x != 0

@@ -65,22 +65,22 @@ __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const
   } else if constexpr (V == V_PULL || V == V_AA_ODD) {
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
-      uint64_t i;
+      const S *ptr;
       if (d == C) {
-        i = slot<false>(cell, C, p.plane);
+        ptr = slotPtr(a, C, p.plane, cell);
       } else {
         const bool bb = solidOf(-cx(d), -cy(d), -cz(d));
         if (V == V_PULL)
-          i = bb ? slot<false>(cell, inv(d), p.plane) : slot<false>(cellOf(-cx(d), -cy(d), -cz(d)), d, p.plane);
+          ptr = bb ? slotPtr(a, inv(d), p.plane, cell) : slotPtr(a, d, p.plane, cellOf(-cx(d), -cy(d), -cz(d)));
         else
-          i = bb ? slot<false>(cell, d, p.plane) : slot<false>(cellOf(-cx(d), -cy(d), -cz(d)), inv(d), p.plane);
+          ptr = bb ? slotPtr(a, d, p.plane, cell) : slotPtr(a, inv(d), p.plane, cellOf(-cx(d), -cy(d), -cz(d)));
       }
-      f[d] = St::load(getS<NTL>(&a[i]), p.inv);
+      f[d] = St::raw(getS<NTL>(ptr));
     }
   } else {
 #pragma unroll
     for (int d = 0; d < Q; ++d)
-      f[d] = St::load(getS<NTL>(&a[slot<false>(cell, d, p.plane)]), p.inv);
+      f[d] = St::raw(getS<NTL>(slotPtr(a, d, p.plane, cell)));
   }
 
   if (p.addForce)
@@ -91,28 +91,24 @@ __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const
 
   if constexpr (V == V_PULL) {
 #pragma unroll
-    for (int d = 0; d < Q; ++d) {
-      const uint64_t i = slot<false>(cell, d, p.plane);
-      putS<NTS>(&b[i], St::storeStep(f[d], p.scale, ditherDir(dz, d)));
-    }
+    for (int d = 0; d < Q; ++d)
+      putS<NTS>(slotPtr(b, d, p.plane, cell), St::rawStep(f[d], ditherDir(dz, d)));
   } else if constexpr (V == V_AA_EVEN) {
 #pragma unroll
-    for (int d = 0; d < Q; ++d) {
-      const uint64_t i = slot<false>(cell, inv(d), p.plane);
-      putS<NTS>(&a[i], St::storeStep(f[d], p.scale, ditherDir(dz, d)));
-    }
+    for (int d = 0; d < Q; ++d)
+      putS<NTS>(slotPtr(a, inv(d), p.plane, cell), St::rawStep(f[d], ditherDir(dz, d)));
   } else {
     S *o = V == V_PUSH ? b : a;
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
-      uint64_t i;
+      S *ptr;
       if (d == C) {
-        i = slot<false>(cell, C, p.plane);
+        ptr = slotPtr(o, C, p.plane, cell);
       } else {
         const bool bb = solidOf(cx(d), cy(d), cz(d));
-        i = bb ? slot<false>(cell, inv(d), p.plane) : slot<false>(cellOf(cx(d), cy(d), cz(d)), d, p.plane);
+        ptr = bb ? slotPtr(o, inv(d), p.plane, cell) : slotPtr(o, d, p.plane, cellOf(cx(d), cy(d), cz(d)));
       }
-      putS<NTS>(&o[i], St::storeStep(f[d], p.scale, ditherDir(dz, d)));
+      putS<NTS>(ptr, St::rawStep(f[d], ditherDir(dz, d)));
     }
   }
 }
@@ -172,7 +168,7 @@ __global__ __launch_bounds__(256) void kStepAoS(const S *a, S *b, const MaskWin 
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
     for (int d = 0; d < Q; ++d)
-      f[d] = fluid ? St::load(st[lane * Q + d], p.inv) : R(0);
+      f[d] = fluid ? St::raw(st[lane * Q + d]) : R(0);
   } else {
     const int xs[3] = { wrapDec(x, p.nx), x, wrapInc(x, p.nx) };
     const int ys[3] = { wrapDec(y, p.ny), y, wrapInc(y, p.ny) };
@@ -193,7 +189,7 @@ __global__ __launch_bounds__(256) void kStepAoS(const S *a, S *b, const MaskWin 
           i = bb ? c0 * Q + (uint64_t)lane * Q + (uint64_t)inv(d)
                  : (rowOf(ox, oy) * p.nz + (uint64_t)zs[oz + 1]) * Q + (uint64_t)d;
         }
-        f[d] = St::load(getS<false>(&a[i]), p.inv);
+        f[d] = St::raw(getS<false>(&a[i]));
       }
     }
   }
@@ -206,7 +202,7 @@ __global__ __launch_bounds__(256) void kStepAoS(const S *a, S *b, const MaskWin 
     if (lane < nvalid) {
 #pragma unroll
       for (int d = 0; d < Q; ++d)
-        st[lane * Q + d] = St::storeStep(f[d], p.scale, ditherDir(dz, d));
+        st[lane * Q + d] = St::rawStep(f[d], ditherDir(dz, d));
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -230,7 +226,7 @@ __global__ __launch_bounds__(256) void kStepAoS(const S *a, S *b, const MaskWin 
         i = bb ? (c0 + (uint64_t)lane) * Q + (uint64_t)inv(d)
                : (rowOf(cx(d), cy(d)) * p.nz + (uint64_t)zs[cz(d) + 1]) * Q + (uint64_t)d;
       }
-      putS<false>(&b[i], St::storeStep(f[d], p.scale, ditherDir(dz, d)));
+      putS<false>(&b[i], St::rawStep(f[d], ditherDir(dz, d)));
     }
   }
 }
@@ -422,7 +418,7 @@ __global__ __launch_bounds__(256, LBM_MIN_WAVES) void kRowSweep(S *a, S *b,
           }
           if (W && d != C && __builtin_amdgcn_inverse_ballot_w64(bitsOf(-cx(d), -cy(d), -cz(d), k)))
             v = getS<NTL>(&a[sl(cell, d)]); // bounce-back: own slot d
-          f[d] = St::load(v, p.inv);
+          f[d] = St::raw(v);
         }
       } else if constexpr (V == V_AA_ODD) {
         if (fluid) {
@@ -434,7 +430,7 @@ __global__ __launch_bounds__(256, LBM_MIN_WAVES) void kRowSweep(S *a, S *b,
               if (W && __builtin_amdgcn_inverse_ballot_w64(bitsOf(-cx(d), -cy(d), -cz(d), k)))
                 i = sl(cell, d);
             }
-            f[d] = St::load(getS<NTL>(&a[i]), p.inv);
+            f[d] = St::raw(getS<NTL>(&a[i]));
           }
         } else {
 #pragma unroll
@@ -444,7 +440,7 @@ __global__ __launch_bounds__(256, LBM_MIN_WAVES) void kRowSweep(S *a, S *b,
       } else if (fluid) {
 #pragma unroll
         for (int d = 0; d < Q; ++d)
-          f[d] = St::load(getS<NTL>(&a[sl(cell, d)]), p.inv);
+          f[d] = St::raw(getS<NTL>(&a[sl(cell, d)]));
       } else {
 #pragma unroll
         for (int d = 0; d < Q; ++d)
@@ -457,7 +453,7 @@ __global__ __launch_bounds__(256, LBM_MIN_WAVES) void kRowSweep(S *a, S *b,
 
 #pragma unroll
       for (int d = 0; d < Q; ++d) {
-        const S v = St::storeStep(f[d], p.scale, ditherDir(dz, d));
+        const S v = St::rawStep(f[d], ditherDir(dz, d));
         if (d == C) {
           if (fluid)
             putS<NTS>(&o[sl(cell, C)], v);

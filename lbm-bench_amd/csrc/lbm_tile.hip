@@ -50,12 +50,12 @@ __global__ __launch_bounds__(256) void kStepTile(S *a, S *b, const MaskWin *__re
         const uint64_t n = nbr(-cx(d), -cy(d), -cz(d), bb);
         i = bb ? slot<false>(cell, inv(d), p.plane) : slot<false>(n, d, p.plane);
       }
-      f[d] = St::load(a[i], p.inv);
+      f[d] = St::raw(a[i]);
     }
   } else {
 #pragma unroll
     for (int d = 0; d < Q; ++d)
-      f[d] = St::load(a[slot<false>(cell, d, p.plane)], p.inv);
+      f[d] = St::raw(a[slot<false>(cell, d, p.plane)]);
   }
   if (p.addForce)
     addForce(f, p.f1, p.f2);
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void kStepTile(S *a, S *b, const MaskWin *__re
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
       const uint64_t i = slot<false>(cell, d, p.plane);
-      b[i] = St::storeStep(f[d], p.scale, ditherDir(dz, d));
+      b[i] = St::rawStep(f[d], ditherDir(dz, d));
     }
   } else {
 #pragma unroll
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(256) void kStepTile(S *a, S *b, const MaskWin *__re
         const uint64_t n = nbr(cx(d), cy(d), cz(d), bb);
         i = bb ? slot<false>(cell, inv(d), p.plane) : slot<false>(n, d, p.plane);
       }
-      b[i] = St::storeStep(f[d], p.scale, ditherDir(dz, d));
+      b[i] = St::rawStep(f[d], ditherDir(dz, d));
     }
   }
 }
