@@ -126,8 +126,10 @@ struct RowCtx {
   uint64_t cell;
   bool fluid;
 
+  // keepSolid: solid lanes stay (they store zeros into their own slots) when
+  // their store line, lineLanes lanes wide, also holds a fluid cell
   __device__ __forceinline__ bool init(const MaskWin *__restrict__ m, int nx, int ny_, int nz_,
-      int segs_, int remap, bool keepSolid = false)
+      int segs_, int remap, bool keepSolid = false, int lineLanes = 64)
   {
     mask = m;
     segs = segs_;
@@ -144,9 +146,16 @@ struct RowCtx {
     if (z >= nz)
       return false;
     const int x = row / ny, y = row - x * ny;
-    fluid       = !bitAt(mask[(uint64_t)row * segs + seg].lo, lane);
-    if (!fluid && !keepSolid)
-      return false;
+    const uint64_t solid = mask[(uint64_t)row * segs + seg].lo;
+    fluid                = !bitAt(solid, lane);
+    if (!fluid) {
+      if (!keepSolid)
+        return false;
+      const uint64_t line = lineLanes >= 64 ? ~0ull
+                                            : (((1ull << lineLanes) - 1ull) << (lane & ~(lineLanes - 1)));
+      if ((~solid & line) == 0ull)
+        return false;
+    }
     xs[0] = wrapDec(x, nx);
     xs[1] = x;
     xs[2] = wrapInc(x, nx);

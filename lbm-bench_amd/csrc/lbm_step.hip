@@ -44,14 +44,16 @@ __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const
   constexpr bool NTS = NT & 1, NTL = NT & 2;
 
   // AA even and pull store only to their own cell, so they keep the lanes
-  // of solid cells: these store the collision of zeros into their own slots
-  // (which no step reads), and a store line shared by fluid and solid cells
-  // is written whole instead of leaving holes for HBM to fill by
-  // read-modify-write (8 % of the FP64 lines of a 512^3 sphere packing,
-  // 22 % of the FP16 ones; measured +2-3 % for FP64 and FP32 AA).
+  // of solid cells whose 128-byte store line holds a fluid cell: these
+  // store the collision of zeros into their own slots (which no step
+  // reads), and a store line shared by fluid and solid cells is written
+  // whole instead of leaving holes for HBM to fill by read-modify-write (8 %
+  // of the FP64 lines of a 512^3 sphere packing, 22 % of the FP16 ones;
+  // measured +2-3 % for FP64 and FP32 AA).  Lines without fluid cells are
+  // not written.
   constexpr bool keepSolid = V == V_AA_EVEN || V == V_PULL;
   RowCtx c;
-  if (!c.init(mask, p.nx, p.ny, p.nz, p.segs, remap, keepSolid))
+  if (!c.init(mask, p.nx, p.ny, p.nz, p.segs, remap, keepSolid, (int)(128 / sizeof(S))))
     return;
   const uint64_t cell = c.cell;
   auto cellOf  = [&](int ox, int oy, int oz) { return c.cellOf(ox, oy, oz); };
