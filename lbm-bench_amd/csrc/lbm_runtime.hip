@@ -333,10 +333,13 @@ static void launchStep(LbmDevice *D, StepLaunch L, int variant, int srcIdx, int 
 {
   L.variant  = variant;
   L.addForce = addForce;
-  // measured at 512^3: the XCD-aware order gains 5 % for the AA even step
-  // of 4-byte elements and loses 1-3 % elsewhere (row sweep, FP16, FP64)
-  if (D->remap < 0)
-    L.remap = variant == V_AA_EVEN && D->esize == 4 ? 1 : 0;
+  // measured at 512^3 (aa-soa): non-temporal loads and stores for 8-byte
+  // elements and for the AA even step of 4- and 2-byte elements (-7 % and
+  // -11 % time against cached loads); the other steps of 4- and 2-byte
+  // elements read lines that neighbouring wavefronts share, and keep their
+  // loads cached (non-temporal there costs 23 % and 27 %)
+  if (D->nt < 0)
+    L.nt = D->esize == 8 || variant == V_AA_EVEN ? 3 : 1;
   L.seqBase  = D->dseq;
   L.seq      = node >= 0 ? (uint32_t)node : eagerSeq(D);
   if (D->prop == PROP_AA) {
@@ -579,12 +582,14 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
   if (D->storage == kNativeStorage)
     D->rhoRef = 0.0;
   D->gsteps  = envInt("LBM_GRAPH_STEPS", 16) & ~1;
-  // XCD-aware block order (LBM_XCD_REMAP 0 / 1; unset: per launch, see
-  // launchStep)
-  D->remap   = envInt("LBM_XCD_REMAP", -1);
-  // measured at 512^3: 8-byte elements stream best with non-temporal loads
-  // and stores; 4- and 2-byte elements with non-temporal stores only
-  D->nt      = envInt("LBM_NT", D->esize == 8 ? 3 : 1);
+  // XCD-aware block order (LBM_XCD_REMAP=1).  Measured at 512^3 (aa-soa)
+  // it loses 1-3 % for the row sweep (FP64, FP16) and 3 % for the FP32 AA
+  // even step with non-temporal loads (it gained 5 % there with cached
+  // loads), so it is off by default.
+  D->remap   = envInt("LBM_XCD_REMAP", 0);
+  // non-temporal hints (LBM_NT: bit 0 stores, bit 1 loads; unset: per
+  // launch, see launchStep)
+  D->nt      = envInt("LBM_NT", -1);
   D->wavesPerBlock = envInt("LBM_WAVES_PER_BLOCK", 4);
   D->dither        = envInt("LBM_DITHER", 1);
   // measured at 512^3 (aa-soa): the row sweep gains 2.5 % with 8-byte and
