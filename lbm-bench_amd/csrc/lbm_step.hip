@@ -372,6 +372,7 @@ __global__ __launch_bounds__(256, LBM_MIN_WAVES) void kRowSweep(S *a, S *b,
     lcarry[d] = S{};
   }
   bool prevClean = true; // block k - 1 had no solid cell near it
+  const uint32_t seqv = stepSeq(p); // read once, not once per block
 
   for (int k = 0; k < K; ++k) {
     const int z  = k * 64 + lane;
@@ -397,29 +398,39 @@ __global__ __launch_bounds__(256, LBM_MIN_WAVES) void kRowSweep(S *a, S *b,
         // the last block), one uniform load.  Each of these slots is read
         // and later written only by a cell of this row that has not stored
         // yet.  All lanes load (a fluid lane may take its value from the lane
-        // of a solid cell); solid cells' results are never stored.  Used for
-        // 8-byte elements only: measured at 512^3, it gains 2 % there, while
-        // the lane shifts cost 2-byte elements 16 %.
+        // of a solid cell); solid cells' results are never stored.  Every
+        // load of the block, bounce-back loads included, is issued before
+        // the first lane shift: a shift right after its own load would make
+        // the wavefront wait for all loads issued so far, once per shifted
+        // direction.  Used for 8-byte elements only: measured at 512^3, the
+        // lane shifts cost 2-byte elements 16 %.
+        S al[Q], edge[Q], own[Q];
 #pragma unroll
         for (int d = 0; d < Q; ++d) {
-          S v;
           if (cz(d) == 0) {
-            v = getS<NTL>(&a[sl(cellN(-cx(d), -cy(d), 0), inv(d))]);
+            al[d] = getS<NTL>(&a[sl(cellN(-cx(d), -cy(d), 0), inv(d))]);
           } else {
             const uint64_t rb = rowStart(-cx(d), -cy(d));
-            if (cz(d) > 0) {
-              const S al   = getS<NTL>(&a[sl(rb + (uint64_t)z, inv(d))]);
-              const S fill = k == 0 ? a[sl(rb + (uint64_t)(nz - 1), inv(d))] : lcarry[d];
-              v            = laneShift(al, fill, true);
-              lcarry[d]    = laneValue(al, 63);
-            } else {
-              const S al   = getS<NTL>(&a[sl(rb + (uint64_t)z, inv(d))]);
-              const int ze = k == K - 1 ? 0 : k * 64 + 64;
-              v            = laneShift(al, a[sl(rb + (uint64_t)ze, inv(d))], false);
-            }
+            al[d]             = getS<NTL>(&a[sl(rb + (uint64_t)z, inv(d))]);
+            if (cz(d) > 0)
+              edge[d] = k == 0 ? a[sl(rb + (uint64_t)(nz - 1), inv(d))] : lcarry[d];
+            else
+              edge[d] = a[sl(rb + (uint64_t)(k == K - 1 ? 0 : k * 64 + 64), inv(d))];
+          }
+          own[d] = S{};
+          if (W && d != C && __builtin_amdgcn_inverse_ballot_w64(bitsOf(-cx(d), -cy(d), -cz(d), k)))
+            own[d] = getS<NTL>(&a[sl(cell, d)]); // bounce-back: own slot d
+        }
+#pragma unroll
+        for (int d = 0; d < Q; ++d) {
+          S v = al[d];
+          if (cz(d) != 0) {
+            v = laneShift(al[d], edge[d], cz(d) > 0);
+            if (cz(d) > 0)
+              lcarry[d] = laneValue(al[d], 63);
           }
           if (W && d != C && __builtin_amdgcn_inverse_ballot_w64(bitsOf(-cx(d), -cy(d), -cz(d), k)))
-            v = getS<NTL>(&a[sl(cell, d)]); // bounce-back: own slot d
+            v = own[d];
           f[d] = St::raw(v);
         }
       } else if constexpr (V == V_AA_ODD) {
@@ -451,7 +462,7 @@ __global__ __launch_bounds__(256, LBM_MIN_WAVES) void kRowSweep(S *a, S *b,
       if (p.addForce)
         addForce(f, p.f1, p.f2);
       collide(f, p.trt);
-      const Dither dz = ditherOf(cell, stepSeq(p), p.dither);
+      const Dither dz = ditherOf(cell, seqv, p.dither);
 
 #pragma unroll
       for (int d = 0; d < Q; ++d) {
