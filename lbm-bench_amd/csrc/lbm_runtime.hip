@@ -333,6 +333,13 @@ static void launchStep(LbmDevice *D, StepLaunch L, int variant, int srcIdx, int 
 {
   L.variant  = variant;
   L.addForce = addForce;
+  // XCD-aware block order: on for the AoS layout, whose scattered side
+  // collects each record line from nine neighbour rows, so rows that share
+  // lines should share an L2 (measured at 256^3: pull-aos +66 %, push-aos
+  // +16 %).  Off for SoA: -7 % pull-soa, -2 % push-soa, -1-3 % row sweep,
+  // -3 % FP32 AA even; blocked variants neutral.
+  if (D->remap < 0)
+    L.remap = D->layout == LAYOUT_AOS ? 1 : 0;
   // measured at 512^3 (aa-soa): non-temporal loads and stores for 8-byte
   // elements and for the AA even step of 4- and 2-byte elements (-7 % and
   // -11 % time against cached loads); the other steps of 4- and 2-byte
@@ -582,11 +589,9 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
   if (D->storage == kNativeStorage)
     D->rhoRef = 0.0;
   D->gsteps  = envInt("LBM_GRAPH_STEPS", 16) & ~1;
-  // XCD-aware block order (LBM_XCD_REMAP=1).  Measured at 512^3 (aa-soa)
-  // it loses 1-3 % for the row sweep (FP64, FP16) and 3 % for the FP32 AA
-  // even step with non-temporal loads (it gained 5 % there with cached
-  // loads), so it is off by default.
-  D->remap   = envInt("LBM_XCD_REMAP", 0);
+  // XCD-aware block order (LBM_XCD_REMAP 0 / 1; unset: per launch, see
+  // launchStep)
+  D->remap   = envInt("LBM_XCD_REMAP", -1);
   // non-temporal hints (LBM_NT: bit 0 stores, bit 1 loads; unset: per
   // launch, see launchStep)
   D->nt      = envInt("LBM_NT", -1);
