@@ -82,12 +82,16 @@ struct LbmDevice {
   double *dpart, *dout;
   int npart;
   int remap, nt, wavesPerBlock, dither, rowSweep;
-  // FP16/BF16 rounding seeds.  dseq (device) counts the steps run by
-  // graphs: each node of a graph adds its index to it and the graph's last
-  // node advances it.  Eager steps and force passes number themselves
-  // from seq, offset by 2^31 so that the two ranges stay apart.
+  // FP16/BF16 rounding seeds.  Every step is seeded with its absolute
+  // number, steps, however the steps are grouped into graph replays and
+  // eager launches, so the rounding does not depend on LBM_GRAPH_STEPS.
+  // A graph node adds its index to the device counter dseq[0], and the
+  // graph's last node advances it; before a replay the host brings it to
+  // `steps` (dseqHost: its value as the host last set it).  Eager steps
+  // add their number to dseq[1], which stays 0.  Force passes number
+  // themselves from seq, offset by 2^31 so that the two ranges stay apart.
   uint32_t *dseq;
-  uint32_t seq;
+  uint32_t seq, steps, dseqHost;
 };
 
 // ---- small helpers ---------------------------------------------------------
@@ -324,7 +328,21 @@ static StepLaunch baseLaunch(const LbmDevice *D, const CaseDataType *cd)
   return L;
 }
 
-static uint32_t eagerSeq(LbmDevice *D) { return 0x80000000u + D->seq++; }
+static uint32_t forceSeq(LbmDevice *D) { return 0x80000000u + D->seq++; }
+
+// a graph replay of G steps, with the device step counter brought to the
+// number of steps run so far first
+static void replaySteps(LbmDevice *D, hipGraphExec_t g, int G)
+{
+  if (D->dseqHost != D->steps) {
+    launchBumpSeq(D->dseq, D->steps - D->dseqHost, D->stream);
+    HIP_OK(hipGetLastError());
+    D->dseqHost = D->steps;
+  }
+  HIP_OK(hipGraphLaunch(g, D->stream));
+  D->steps += (uint32_t)G;
+  D->dseqHost += (uint32_t)G;
+}
 
 // one step; for two-lattice variants src is D->d[srcIdx].  node >= 0 is the
 // step's index inside a graph being captured, -1 an eager launch.
@@ -347,8 +365,13 @@ static void launchStep(LbmDevice *D, StepLaunch L, int variant, int srcIdx, int 
   // loads cached (non-temporal there costs 23 % and 27 %)
   if (D->nt < 0)
     L.nt = D->esize == 8 || variant == V_AA_EVEN ? 3 : 1;
-  L.seqBase  = D->dseq;
-  L.seq      = node >= 0 ? (uint32_t)node : eagerSeq(D);
+  if (node >= 0) {
+    L.seqBase = D->dseq;
+    L.seq     = (uint32_t)node;
+  } else {
+    L.seqBase = D->dseq + 1;
+    L.seq     = D->steps++;
+  }
   if (D->prop == PROP_AA) {
     L.src = L.dst = D->d[0];
   } else {
@@ -447,7 +470,7 @@ static void runKernel(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cdI
       launchStep(D, base, V_AA_ODD, 0, bf, D->stream);
       left -= 2;
       while (G > 0 && left >= G) {
-        HIP_OK(hipGraphLaunch(D->gexec[0], D->stream));
+        replaySteps(D, D->gexec[0], G);
         left -= G;
       }
       while (left > 0) {
@@ -460,7 +483,7 @@ static void runKernel(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cdI
       D->active ^= 1;
       left -= 1;
       while (G > 0 && left >= G) {
-        HIP_OK(hipGraphLaunch(D->gexec[D->active], D->stream));
+        replaySteps(D, D->gexec[D->active], G);
         left -= G; // G is even: the state ends in the same array
       }
       while (left > 0) {
@@ -472,7 +495,7 @@ static void runKernel(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cdI
     if (bf) {
       const int rule = D->prop == PROP_PULL ? SR_PULL : SR_LOCAL;
       AuxLaunch A    = auxOf(D, rule);
-      A.seq          = eagerSeq(D);
+      A.seq          = forceSeq(D);
       launchForce(A, base.f1, base.f2, D->stream);
       HIP_OK(hipGetLastError());
     }
@@ -601,6 +624,8 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
   // 11-13 % with 2-byte elements and loses 4 % with 4-byte elements
   D->rowSweep      = envInt("LBM_ROW_SWEEP", D->esize == 4 ? 0 : 1);
   D->seq           = 0;
+  D->steps         = 0;
+  D->dseqHost      = 0;
   kd->LoopBalance = 2.0 * Q * (double)D->esize;
 
   HIP_OK(hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking));
@@ -714,7 +739,7 @@ void kernelAddBodyForce(KernelDataType *kd, LatticeDescType *ld, CaseDataType *c
   flushMirror(D);
   const StepLaunch base = baseLaunch(D, cd);
   AuxLaunch A = auxOf(D, currentRule(D, kd));
-  A.seq       = eagerSeq(D);
+  A.seq       = forceSeq(D);
   launchForce(A, base.f1, base.f2, D->stream);
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(D->stream));

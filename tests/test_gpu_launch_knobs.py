@@ -1,0 +1,64 @@
+"""Launch shape does not change results.
+
+Every launch knob (block order across XCDs, non-temporal hints, the row
+sweep against the per-block kernel, wavefronts per workgroup, plane
+padding) picks another way to run the same arithmetic on the same slots,
+and the stochastic rounding seeds depend only on cell and step.  So each
+setting must give the default run's state bit for bit, for FP64 and for
+FP16 storage, on a sphere channel with several 64-cell blocks per row
+(walls, solids inside the rows, the z wrap) and the body force on.
+LBM_ROW_SWEEP swaps kernels (kRowSweep / kStepRow), whose compilations
+fuse multiply-adds of the collision differently: there the bound is 1e-14
+relative (FP64) or a few FP16 quanta of the stored deviation (FP16)."""
+import numpy as np
+import pytest
+
+from lbmhip import Lattice
+
+pytestmark = pytest.mark.gpu
+
+DIMS = (14, 16, 128)
+KNOBS = [
+    {"LBM_XCD_REMAP": "1"},
+    {"LBM_XCD_REMAP": "0"},
+    {"LBM_NT": "0"},
+    {"LBM_NT": "1"},
+    {"LBM_NT": "2"},
+    {"LBM_NT": "3"},
+    {"LBM_ROW_SWEEP": "0"},
+    {"LBM_ROW_SWEEP": "1"},
+    {"LBM_WAVES_PER_BLOCK": "8"},
+    {"LBM_PLANE_PAD": "4096"},
+    {"LBM_GRAPH_STEPS": "4"},
+]
+
+
+def _state(lib, kernel, storage, steps=7):
+    with Lattice(lib, kernel, "spheres-6", DIMS, (0, 0, 1), omega=1.25, xforce=1e-5,
+                 body_force=True, magic=3 / 16, storage=storage) as L:
+        L.init("random", 9, 1.0, 1e-3)
+        L.run(steps)
+        return L.get_nodes()
+
+
+@pytest.mark.parametrize("storage", ["native", "fp16"])
+@pytest.mark.parametrize("kernel", ["aa-soa", "push-soa", "pull-soa", "pull-aos", "push-aos",
+                                    "blk-push-soa", "blk-pull-soa"])
+def test_knobs_leave_results_bit_identical(gpu_lib, monkeypatch, kernel, storage):
+    ref = _state(gpu_lib, kernel, storage)
+    assert np.isfinite(ref).all()
+    for knob in KNOBS:
+        with monkeypatch.context() as m:
+            for k, v in knob.items():
+                m.setenv(k, v)
+            got = _state(gpu_lib, kernel, storage)
+        if "LBM_ROW_SWEEP" in knob:
+            fluid = ref != 0
+            if storage == "native":
+                err = float(np.abs(got - ref)[fluid].max() / np.abs(ref[fluid]).max())
+                assert err < 1e-14, (kernel, storage, knob, err)
+            else:
+                assert float(np.abs(got - ref).max()) < 1e-5, (kernel, storage, knob)
+            continue
+        same = np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+        assert same, (kernel, storage, knob, float(np.abs(got - ref).max()))
