@@ -243,8 +243,8 @@ __device__ __forceinline__ bool bitAt(uint64_t word, int lane)
   return (word >> lane) & 1ull;
 }
 
-// per-lane lookup of an arbitrary cell (used by the tiled, AoS-free
-// generic passes: init, force, reductions)
+// per-lane lookup of an arbitrary cell (used by the generic passes: init,
+// force, reductions, the adjoint objective)
 __device__ __forceinline__ bool solidAt(const MaskWin *__restrict__ mask, int segs, int nY,
     int x, int y, int z)
 {

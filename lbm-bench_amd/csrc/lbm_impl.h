@@ -1,5 +1,5 @@
 // lbm_impl.h -- kernel-side parameter block and slot addressing shared by
-// lbm_step.hip, lbm_tile.hip and lbm_aux.hip.
+// lbm_step.hip and lbm_aux.hip.
 #pragma once
 
 #include "lbm_kernels.h"
@@ -127,21 +127,34 @@ struct RowCtx {
   bool fluid;
 
   // keepSolid: solid lanes stay (they store zeros into their own slots) when
-  // their store line, lineLanes lanes wide, also holds a fluid cell
+  // their store line, lineLanes lanes wide, also holds a fluid cell.
+  // tiled: rows are taken in tiles of 2 x 2 (x, y), tiles y fastest, so
+  // that the wavefronts of consecutive rows of the grid cover a 2 x 2 x nZ
+  // block; otherwise row after row.  Wavefront w of the grid owns segment
+  // w % segs of the (w / segs)-th row in that order.
   __device__ __forceinline__ bool init(const MaskWin *__restrict__ m, int nx, int ny_, int nz_,
-      int segs_, int remap, bool keepSolid = false, int lineLanes = 64)
+      int segs_, int remap, bool keepSolid = false, int lineLanes = 64, int tiled = 0)
   {
     mask = m;
     segs = segs_;
     ny   = ny_;
     nz   = nz_;
     const unsigned blk = remap ? xcdLogicalBlock(blockIdx.x, gridDim.x) : blockIdx.x;
-    const int wv = __builtin_amdgcn_readfirstlane((int)(blk * (blockDim.x >> 6) + (threadIdx.x >> 6)));
     lane = (int)(threadIdx.x & 63u);
-    row  = wv / segs;
-    if (row >= nx * ny)
-      return false;
-    seg = wv - row * segs;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(blk * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+    const int r  = wv / segs;
+    seg          = wv - r * segs;
+    if (tiled) {
+      const int tY = (ny + 1) / 2, t = r >> 2, q = r & 3;
+      const int xr = (t / tY) * 2 + (q >> 1), yr = (t % tY) * 2 + (q & 1);
+      if (xr >= nx || yr >= ny)
+        return false;
+      row = xr * ny + yr;
+    } else {
+      row = r;
+      if (row >= nx * ny)
+        return false;
+    }
     z   = seg * 64 + lane;
     if (z >= nz)
       return false;
@@ -190,6 +203,11 @@ inline unsigned rowBlocks(const Geom &g, int wavesPerBlock)
 {
   const uint64_t wv = (uint64_t)g.nx * g.ny * g.segs;
   return (unsigned)((wv + wavesPerBlock - 1) / wavesPerBlock);
+}
+// tiles of 2 x 2 rows (4 wavefronts each) of a tiled launch, times segs
+inline unsigned tileBlocks(const Geom &g, int segs)
+{
+  return (unsigned)((uint64_t)((g.nx + 1) / 2) * (uint64_t)((g.ny + 1) / 2) * (uint64_t)segs);
 }
 
 } // namespace lbm

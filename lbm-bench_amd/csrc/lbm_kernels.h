@@ -1,6 +1,6 @@
 // lbm_kernels.h -- launch interface between the host runtime
 // (lbm_runtime.hip) and the kernel translation units (lbm_step.hip,
-// lbm_tile.hip, lbm_aux.hip).  Plain structs, no templates, so each kernel
+// lbm_aux.hip).  Plain structs, no templates, so each kernel
 // file instantiates its own templates and the runtime stays small.
 #pragma once
 
@@ -51,7 +51,6 @@ struct StepLaunch {
 };
 
 void launchStepRow(const StepLaunch &L, hipStream_t s);  // lbm_step.hip
-void launchStepTile(const StepLaunch &L, hipStream_t s); // lbm_tile.hip
 
 // auxiliary passes over every fluid cell (lbm_aux.hip)
 struct AuxLaunch {

@@ -378,10 +378,7 @@ static void launchStep(LbmDevice *D, StepLaunch L, int variant, int srcIdx, int 
     L.src = D->d[srcIdx];
     L.dst = D->d[srcIdx ^ 1];
   }
-  if (D->tiled)
-    launchStepTile(L, s);
-  else
-    launchStepRow(L, s);
+  launchStepRow(L, s); // L.tiled orders the blocked variants' wavefronts in tiles
   HIP_OK(hipGetLastError());
 }
 
@@ -568,7 +565,7 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
     kd->Dims[a]       = ld->Dims[a];
     kd->GlobalDims[a] = ld->Dims[a];
     kd->Offsets[a]    = 0;
-    kd->Blk[a]        = tiled ? (a == 2 ? 64 : 2) : ld->Dims[a]; // the tile of lbm_tile.hip
+    kd->Blk[a]        = tiled ? (a == 2 ? 64 : 2) : ld->Dims[a]; // 2 x 2 rows, 64-cell segments
   }
   kd->Iteration = -1;
   kd->Kernel    = kernel;

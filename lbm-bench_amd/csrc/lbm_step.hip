@@ -37,7 +37,7 @@ namespace lbm {
 
 template <int V, int NT, typename S>
 __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const MaskWin *__restrict__ mask,
-    Params<S> p, int remap)
+    Params<S> p, int remap, int tiled)
 {
   using R = typename Store<S>::R;
   using St = Store<S>;
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const
   // not written.
   constexpr bool keepSolid = V == V_AA_EVEN || V == V_PULL;
   RowCtx c;
-  if (!c.init(mask, p.nx, p.ny, p.nz, p.segs, remap, keepSolid, (int)(128 / sizeof(S))))
+  if (!c.init(mask, p.nx, p.ny, p.nz, p.segs, remap, keepSolid, (int)(128 / sizeof(S)), tiled))
     return;
   const uint64_t cell = c.cell;
   auto cellOf  = [&](int ox, int oy, int oz) { return c.cellOf(ox, oy, oz); };
@@ -317,7 +317,7 @@ __device__ __forceinline__ S laneValue(S v, int lane)
 
 template <int V, int NT, typename S>
 __global__ __launch_bounds__(256, LBM_MIN_WAVES) void kRowSweep(S *a, S *b,
-    const MaskWin *__restrict__ mask, Params<S> p, int remap)
+    const MaskWin *__restrict__ mask, Params<S> p, int remap, int tiled)
 {
   using R  = typename Store<S>::R;
   using St = Store<S>;
@@ -325,9 +325,20 @@ __global__ __launch_bounds__(256, LBM_MIN_WAVES) void kRowSweep(S *a, S *b,
   static_assert(V == V_PUSH || V == V_AA_ODD, "row sweep: push and AA odd only");
 
   const unsigned blk = remap ? xcdLogicalBlock(blockIdx.x, gridDim.x) : blockIdx.x;
-  const int row = __builtin_amdgcn_readfirstlane((int)(blk * (blockDim.x >> 6) + (threadIdx.x >> 6)));
-  if (row >= p.nrows)
-    return;
+  // tiled: the workgroup's 4 wavefronts are a tile of 2 x 2 rows (y faster)
+  int row;
+  if (tiled) {
+    const int w       = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const unsigned tY = (unsigned)(p.ny + 1) / 2u;
+    const int xr = (int)(blk / tY) * 2 + (w >> 1), yr = (int)(blk % tY) * 2 + (w & 1);
+    if (xr >= p.nx || yr >= p.ny)
+      return;
+    row = __builtin_amdgcn_readfirstlane(xr * p.ny + yr);
+  } else {
+    row = __builtin_amdgcn_readfirstlane((int)(blk * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+    if (row >= p.nrows)
+      return;
+  }
   const int lane = (int)(threadIdx.x & 63u);
   const int nz = p.nz, K = p.segs;
   const int x = row / p.ny, y = row - x * p.ny;
@@ -535,9 +546,9 @@ template <int V, int NT, typename S>
 static void launchRowSweep(const StepLaunch &L, hipStream_t s)
 {
   Params<S> p           = makeParams<S>(L);
-  const unsigned blocks = (unsigned)(((uint64_t)L.g.nx * L.g.ny + 3) / 4);
+  const unsigned blocks = L.tiled ? tileBlocks(L.g, 1) : (unsigned)(((uint64_t)L.g.nx * L.g.ny + 3) / 4);
   hipLaunchKernelGGL((kRowSweep<V, NT, S>), dim3(blocks), dim3(256), 0, s, (S *)L.src,
-      (S *)L.dst, L.g.mask, p, L.remap);
+      (S *)L.dst, L.g.mask, p, L.remap, L.tiled);
 }
 
 template <int V, bool AOS, int NT, typename S>
@@ -551,10 +562,10 @@ static void launchT(const StepLaunch &L, hipStream_t s)
         return launchRowSweep<V, NT, S>(L, s);
     }
     Params<S> p       = makeParams<S>(L);
-    const int wpb     = L.wavesPerBlock == 8 ? 8 : 4;
-    const unsigned blocks = rowBlocks(L.g, wpb);
+    const int wpb     = L.tiled ? 4 : (L.wavesPerBlock == 8 ? 8 : 4);
+    const unsigned blocks = L.tiled ? tileBlocks(L.g, L.g.segs) : rowBlocks(L.g, wpb);
     hipLaunchKernelGGL((kStepRow<V, NT, S>), dim3(blocks), dim3(64 * wpb), 0, s, (S *)L.src,
-        (S *)L.dst, L.g.mask, p, L.remap);
+        (S *)L.dst, L.g.mask, p, L.remap, L.tiled);
   }
 }
 
