@@ -138,9 +138,26 @@ def cpu_reference_openmp():
                       f"{float(m.group(2)):.1f} s"}
 
 
+def ref_cpu_mlups(kernel, n, iters, threads):
+    """LBM-Bench's own binary (oracle/_ref/lbmbench-ref) for one variant on
+    an n^3 channel: its printed MFLUP/s, or None where it is not built"""
+    exe = os.path.join(ROOT, "oracle", "_ref", "lbmbench-ref")
+    if not os.path.exists(exe):
+        return None
+    cmd = [exe, "-d", f"{n}x{n}x{n}", "-g", "channel", "-z", "-k", kernel, "-i", str(iters),
+           "-o", "1.25", "-f", "1e-6"]
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PLACES="cores",
+               OMP_PROC_BIND="close")
+    out = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=900).stdout
+    m = re.search(r"^P:\s+([0-9.]+) MFLUP/s", out, flags=re.M)
+    return float(m.group(1)) if m else None
+
+
 def study_variants(lib, lbmhip, n, steps):
     """the seven variants on an n^3 channel (walls at y, periodic x and z),
-    f = w(1+eps) from seed 0, FP64: MLUPS and GB/s each"""
+    f = w(1+eps) from seed 0, FP64: MLUPS and GB/s each, beside LBM-Bench's
+    own CPU kernels on the host (OpenMP where the reference has it: push-soa
+    and pull-soa; the other five are serial there)"""
     dims = (n, n, n)
     kw = dict(omega=1.25, xforce=1e-6, body_force=True, magic=3.0 / 16.0)
     rows = {}
@@ -151,12 +168,17 @@ def study_variants(lib, lbmhip, n, steps):
         dur = L.run(steps)
         ml = L.n_fluid * steps / dur / 1e6
         gbs = BYTES_PER_LUP["native"] * ml / 1e3
-        rows[kernel] = {"mlups": round(ml, 1), "gb_s": round(gbs, 1),
-                        "frac_of_8tb": round(gbs / HBM_PEAK_GBS, 4)}
-        log(f"{kernel}: {ml:.0f} MLUPS, {gbs:.0f} GB/s")
         L.close()
+        threads = min(16, os.cpu_count() or 1) if kernel in ("push-soa", "pull-soa") else 1
+        cpu = ref_cpu_mlups(kernel, n, 40 if threads > 1 else 8, threads)
+        rows[kernel] = {"mlups": round(ml, 1), "gb_s": round(gbs, 1),
+                        "frac_of_8tb": round(gbs / HBM_PEAK_GBS, 4),
+                        "cpu_reference_mlups": cpu, "cpu_threads": threads}
+        log(f"{kernel}: {ml:.0f} MLUPS, {gbs:.0f} GB/s; LBM-Bench on {threads} host "
+            f"thread(s): {cpu} MLUPS")
     return {"study": "the seven variants", "workload": f"{n}^3 channel -z, omega 1.25, Lambda "
-            f"3/16, body force 1e-6, f = w(1+eps), {steps} steps, FP64", "variants": rows}
+            f"3/16, body force 1e-6, f = w(1+eps), {steps} steps, FP64", "cpu_model": cpu_model(),
+            "variants": rows}
 
 
 def study_ceiling(lib, n):
