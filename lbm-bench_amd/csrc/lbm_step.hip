@@ -1,15 +1,19 @@
-// lbm_step.hip -- the stream-and-collide step of the row-ordered variants
-// (LBM-Bench src/kernel_{pull,push,aa}_soa.c, src/kernel_{pull,push}_aos.c).
+// lbm_step.hip -- the stream-and-collide step of the seven variants
+// (LBM-Bench src/kernel_{pull,push,aa}_soa.c, src/kernel_{pull,push}_aos.c,
+// src/kernel_blk_{pull,push}_soa.c).
 //
 // Three kernels share the slot rules below:
 //   kStepRow   SoA, one wavefront per 64 consecutive z of one (x, y) row
-//              (wave w -> row w / segs, segment w % segs), 4 per workgroup;
-//              pull, push, AA even and AA odd.
+//              (wave w -> segment w % segs of the (w / segs)-th row), 4 per
+//              workgroup; pull, push, AA even and AA odd.
 //   kRowSweep  SoA push and AA odd with nZ % 64 == 0: one wavefront walks a
 //              whole row so that its z-shifted stores (and, for 8-byte
 //              elements, loads) are line-aligned through DPP lane shifts.
 //   kStepAoS   AoS push and pull: the side that touches the wavefront's own
 //              64 records goes through LDS.
+// The blocked variants run kStepRow / kRowSweep with the rows taken in
+// tiles of 2 x 2 (x, y) instead of row after row (`tiled`); every cell is
+// independent, so the results equal the unblocked step.
 // Row, segment and the nine neighbouring rows are wavefront-uniform, so the
 // mask windows of the 3x3 row neighbourhood come in as scalar loads and
 // each lane picks its z-1 / z / z+1 solid bits by shifting them.
