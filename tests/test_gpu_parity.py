@@ -77,6 +77,35 @@ def test_matches_oracle(gpu_lib, oracle, geo, dims, per, kernel):
             assert rel_err(L.get_nodes(), exp, fluid) < TOL
 
 
+# degenerate shapes: an axis of one or two cells, whose neighbours wrap onto
+# the cell itself or its one partner, a single fluid row between two walls,
+# and the row sweep (nZ = 64, 128) on rows that are their own x neighbours
+SMALL = [
+    ("fluid", (1, 4, 5), (1, 1, 1)),
+    ("fluid", (2, 1, 7), (1, 1, 1)),
+    ("fluid", (3, 5, 1), (1, 1, 1)),
+    ("fluid", (2, 2, 2), (1, 1, 1)),
+    ("fluid", (1, 1, 1), (1, 1, 1)),
+    ("channel", (2, 3, 64), (0, 0, 1)),
+    ("channel", (1, 5, 128), (0, 0, 1)),
+]
+
+
+@pytest.mark.parametrize("geo,dims,per", SMALL)
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_degenerate_shapes_match_oracle(gpu_lib, oracle, geo, dims, per, kernel):
+    lat, per_o, _ = oracle.geometry(geo, dims, per, 3)
+    init = oracle.init_random(dims, lat, 21, 1e-2)
+    fluid = lat.transpose(2, 1, 0) != 0
+    assert fluid.any()
+    with Lattice(gpu_lib, kernel, geo, dims, per, omega=1.4, xforce=1e-4, body_force=True,
+                 magic=3 / 16, seed=3) as L:
+        L.set_nodes(init)
+        L.run(5)
+        exp, _ = oracle.run(kernel, dims, lat, per_o, init, 5, 1.4, 3 / 16, 1e-4, True)
+        assert rel_err(L.get_nodes(), exp, fluid) < TOL
+
+
 @pytest.mark.parametrize("kernel", KERNELS)
 def test_split_runs_equal_one_run(gpu_lib, kernel):
     dims = (12, 14, 16)
