@@ -716,12 +716,12 @@ void kernelInitPullAoS(LatticeDescType *ld, KernelDataType **kd, CaseDataType *c
 void kernelInitBlkPushSoA(LatticeDescType *ld, KernelDataType **kd, CaseDataType *cd)
 {
   initInternal(ld, kd, cd, PROP_PUSH, LAYOUT_SOA, 1, "blk-push-soa", kernelBlkPushSoA);
-  printf("# blocking x: %3d y: %3d z: %3d\n", 8, 8, 8);
+  printf("# blocking x: %3d y: %3d z: %3d\n", (*kd)->Blk[0], (*kd)->Blk[1], (*kd)->Blk[2]);
 }
 void kernelInitBlkPullSoA(LatticeDescType *ld, KernelDataType **kd, CaseDataType *cd)
 {
   initInternal(ld, kd, cd, PROP_PULL, LAYOUT_SOA, 1, "blk-pull-soa", kernelBlkPullSoA);
-  printf("# blocking x: %3d y: %3d z: %3d\n", 8, 8, 8);
+  printf("# blocking x: %3d y: %3d z: %3d\n", (*kd)->Blk[0], (*kd)->Blk[1], (*kd)->Blk[2]);
 }
 void kernelInitAaSoA(LatticeDescType *ld, KernelDataType **kd, CaseDataType *cd)
 {
