@@ -145,7 +145,10 @@ size_t lbmAdjointDeviceBytes(LbmAdjoint *adj);
 /* Porous solid-fraction field: u ~ U[0,1) per cell from the splitmix64
  * chain over (seed, cell), Gaussian-smoothed with standard deviation sigma
  * (periodic, radius 3 sigma), standardised to zero mean and unit variance
- * (z), then s = 1 / (1 + exp(-(a z + b))).  Output float64 [x][y][z]. */
+ * (z), then s = 1 / (1 + exp(-(a z + b))).  Output float64 [x][y][z] in
+ * host memory.  Noise, smoothing and the logistic run on the current
+ * device (temporary buffers of 2 n doubles); mean and variance are summed
+ * on the host in cell order. */
 void lbmPorousField(const int dims[3], uint64_t seed, double sigma, double a, double b,
     double *s);
 

@@ -493,6 +493,80 @@ void launchCopy(const double *src, double *dst, uint64_t n, hipStream_t s)
   hipLaunchKernelGGL(kCopy, dim3(blocks), dim3(256), 0, s, (const v2d *)src, (v2d *)dst, n2);
 }
 
+// ---- porous solid-fraction field (differentiation form) ---------------------
+// The smoothing passes compute each output as the host formula does, tap by
+// tap from -rad to rad without fused multiply-adds, so the smoothed field
+// is the one a serial host loop gives, bit for bit.
+__global__ __launch_bounds__(256) void kPorousNoise(double *s, uint64_t n, uint64_t seed)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  if (i < n)
+    s[i] = (double)(hash3(seed, i, 0) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// one axis of the separable periodic Gaussian, field [x][y][z]
+template <int AXIS>
+__global__ __launch_bounds__(256) void kSmoothAxis(const double *__restrict__ in,
+    double *__restrict__ out, int nx, int ny, int nz, const double *__restrict__ kern, int rad)
+{
+#pragma clang fp contract(off)
+  const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  if (i >= (uint64_t)nx * ny * nz)
+    return;
+  const int z = (int)(i % (uint64_t)nz);
+  const int y = (int)((i / (uint64_t)nz) % (uint64_t)ny);
+  const int x = (int)(i / ((uint64_t)nz * (uint64_t)ny));
+  const int n      = AXIS == 0 ? nx : (AXIS == 1 ? ny : nz);
+  const int p      = AXIS == 0 ? x : (AXIS == 1 ? y : z);
+  const int64_t st = AXIS == 0 ? (int64_t)ny * nz : (AXIS == 1 ? (int64_t)nz : 1);
+  const int64_t base = (int64_t)i - (int64_t)p * st;
+  double acc = 0.0;
+  for (int k = -rad; k <= rad; ++k) {
+    int q = (p + k) % n;
+    q     = q < 0 ? q + n : q;
+    acc += kern[k + rad] * in[base + (int64_t)q * st];
+  }
+  out[i] = acc;
+}
+
+__global__ __launch_bounds__(256) void kLogistic(double *s, uint64_t n, double mean, double sd,
+    double a, double b)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  if (i >= n)
+    return;
+  const double z = sd > 0.0 ? (s[i] - mean) / sd : 0.0;
+  s[i]           = 1.0 / (1.0 + exp(-(a * z + b)));
+}
+
+void launchPorousNoise(double *s, uint64_t n, uint64_t seed, hipStream_t st)
+{
+  hipLaunchKernelGGL(kPorousNoise, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s, n, seed);
+}
+
+void launchSmoothAxis(int axis, const double *in, double *out, const int dims[3],
+    const double *kern, int rad, hipStream_t st)
+{
+  const uint64_t n      = (uint64_t)dims[0] * dims[1] * dims[2];
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  if (axis == 0)
+    hipLaunchKernelGGL(kSmoothAxis<0>, dim3(blocks), dim3(256), 0, st, in, out, dims[0], dims[1],
+        dims[2], kern, rad);
+  else if (axis == 1)
+    hipLaunchKernelGGL(kSmoothAxis<1>, dim3(blocks), dim3(256), 0, st, in, out, dims[0], dims[1],
+        dims[2], kern, rad);
+  else
+    hipLaunchKernelGGL(kSmoothAxis<2>, dim3(blocks), dim3(256), 0, st, in, out, dims[0], dims[1],
+        dims[2], kern, rad);
+}
+
+void launchLogistic(double *s, uint64_t n, double mean, double sd, double a, double b,
+    hipStream_t st)
+{
+  hipLaunchKernelGGL(kLogistic, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s, n, mean, sd,
+      a, b);
+}
+
 // ---- access-pattern ceiling --------------------------------------------------
 // The memory traffic of one SoA step without its arithmetic: lane `cell`
 // loads element `cell` of P planes (non-temporal) and stores each value

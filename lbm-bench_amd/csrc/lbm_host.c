@@ -390,66 +390,3 @@ void kernelVerify(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cdIn, P
   printf("# Kernel validation: L2 error norm of relative error: %e\n", (double)*errorNorm);
   free(prof);
 }
-
-/* ------------------------------------------------------------------------ */
-/* porous solid-fraction field for the differentiation form                   */
-/* ------------------------------------------------------------------------ */
-static void smoothAxis(const double *in, double *out, const int dims[3], int axis,
-    const double *kern, int rad)
-{
-  const int nx = dims[0], ny = dims[1], nz = dims[2];
-  const int n[3] = { nx, ny, nz };
-  for (int x = 0; x < nx; ++x)
-    for (int y = 0; y < ny; ++y)
-      for (int z = 0; z < nz; ++z) {
-        double acc = 0.0;
-        int p[3]   = { x, y, z };
-        for (int k = -rad; k <= rad; ++k) {
-          int q[3] = { p[0], p[1], p[2] };
-          q[axis]  = ((p[axis] + k) % n[axis] + n[axis]) % n[axis];
-          acc += kern[k + rad] * in[((size_t)q[0] * ny + q[1]) * nz + q[2]];
-        }
-        out[((size_t)x * ny + y) * nz + z] = acc;
-      }
-}
-
-void lbmPorousField(const int dims[3], uint64_t seed, double sigma, double a, double b,
-    double *s)
-{
-  const size_t n = (size_t)dims[0] * dims[1] * dims[2];
-  double *t      = (double *)malloc(n * sizeof(double));
-  if (t == NULL) {
-    fprintf(stderr, "Error: Insufficient memory to fulfill the request\n");
-    exit(EXIT_FAILURE);
-  }
-  for (size_t i = 0; i < n; ++i)
-    s[i] = hashU01(seed, (uint64_t)i, 0);
-  if (sigma > 0.0) {
-    const int rad = (int)ceil(3.0 * sigma);
-    double *kern  = (double *)malloc(sizeof(double) * (size_t)(2 * rad + 1));
-    double norm   = 0.0;
-    for (int k = -rad; k <= rad; ++k) {
-      kern[k + rad] = exp(-0.5 * (double)k * k / (sigma * sigma));
-      norm += kern[k + rad];
-    }
-    for (int k = 0; k <= 2 * rad; ++k)
-      kern[k] /= norm;
-    smoothAxis(s, t, dims, 0, kern, rad);
-    smoothAxis(t, s, dims, 1, kern, rad);
-    smoothAxis(s, t, dims, 2, kern, rad);
-    memcpy(s, t, n * sizeof(double));
-    free(kern);
-  }
-  double mean = 0.0, var = 0.0;
-  for (size_t i = 0; i < n; ++i)
-    mean += s[i];
-  mean /= (double)n;
-  for (size_t i = 0; i < n; ++i)
-    var += (s[i] - mean) * (s[i] - mean);
-  const double sd = sqrt(var / (double)n);
-  for (size_t i = 0; i < n; ++i) {
-    const double z = sd > 0.0 ? (s[i] - mean) / sd : 0.0;
-    s[i]           = 1.0 / (1.0 + exp(-(a * z + b)));
-  }
-  free(t);
-}

@@ -84,6 +84,14 @@ int launchCompareFields(const double *fa, const double *fb, uint64_t ncells, dou
 void launchReduceCompare(const double *partials, int n, double *out, hipStream_t s);
 // plain device copy, for the measured bandwidth line and counter scaling
 void launchCopy(const double *src, double *dst, uint64_t n, hipStream_t s);
+// porous solid-fraction field: uniform noise from the splitmix64 chain over
+// (seed, cell), one periodic Gaussian pass per axis (field [x][y][z], taps
+// kern[0 .. 2 rad]), and s = 1 / (1 + exp(-(a (s - mean) / sd + b)))
+void launchPorousNoise(double *s, uint64_t n, uint64_t seed, hipStream_t st);
+void launchSmoothAxis(int axis, const double *in, double *out, const int dims[3],
+    const double *kern, int rad, hipStream_t st);
+void launchLogistic(double *s, uint64_t n, double mean, double sd, double a, double b,
+    hipStream_t st);
 // the step's memory traffic without arithmetic over `planes` planes of
 // `cells` elements of `elementBytes` bytes (planes 1, 2, 4, 8 or 19;
 // elementBytes 2, 4 or 8; false for any other); src == dst when in place

@@ -1070,6 +1070,60 @@ double lbmCopyBandwidth(size_t bytes, int reps)
   return 2.0 * (double)(n / 2 * 2) * sizeof(double) * reps / ((double)ms * 1e-3) / 1e9;
 }
 
+void lbmPorousField(const int dims[3], uint64_t seed, double sigma, double a, double b,
+    double *s)
+{
+  // noise and smoothing on the device; the mean and the variance are summed
+  // on the host in cell order (so the field does not depend on a device
+  // reduction's order); the logistic on the device
+  const uint64_t n   = (uint64_t)dims[0] * dims[1] * dims[2];
+  const size_t bytes = n * sizeof(double);
+  double *u = nullptr, *t = nullptr, *kd = nullptr;
+  hipStream_t st;
+  HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  HIP_OK(hipMalloc(&u, bytes));
+  HIP_OK(hipMalloc(&t, bytes));
+  launchPorousNoise(u, n, seed, st);
+  HIP_OK(hipGetLastError());
+  double *field = u;
+  if (sigma > 0.0) {
+    const int rad = (int)ceil(3.0 * sigma);
+    std::vector<double> kern((size_t)(2 * rad + 1));
+    double norm = 0.0;
+    for (int k = -rad; k <= rad; ++k) {
+      kern[(size_t)(k + rad)] = exp(-0.5 * (double)k * k / (sigma * sigma));
+      norm += kern[(size_t)(k + rad)];
+    }
+    for (double &w : kern)
+      w /= norm;
+    HIP_OK(hipMalloc(&kd, kern.size() * sizeof(double)));
+    HIP_OK(hipMemcpyAsync(kd, kern.data(), kern.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    launchSmoothAxis(0, u, t, dims, kd, rad, st);
+    launchSmoothAxis(1, t, u, dims, kd, rad, st);
+    launchSmoothAxis(2, u, t, dims, kd, rad, st);
+    HIP_OK(hipGetLastError());
+    field = t;
+  }
+  HIP_OK(hipMemcpyAsync(s, field, bytes, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  double mean = 0.0, var = 0.0;
+  for (uint64_t i = 0; i < n; ++i)
+    mean += s[i];
+  mean /= (double)n;
+  for (uint64_t i = 0; i < n; ++i)
+    var += (s[i] - mean) * (s[i] - mean);
+  const double sd = sqrt(var / (double)n);
+  launchLogistic(field, n, mean, sd, a, b, st);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(s, field, bytes, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  if (kd)
+    HIP_OK(hipFree(kd));
+  HIP_OK(hipFree(u));
+  HIP_OK(hipFree(t));
+  HIP_OK(hipStreamDestroy(st));
+}
+
 double lbmPatternBandwidth(int planes, int elementBytes, int inPlace, size_t cells, int reps)
 {
   const bool okPlanes = planes == 1 || planes == 2 || planes == 4 || planes == 8 || planes == 19;
