@@ -148,7 +148,8 @@ size_t lbmAdjointDeviceBytes(LbmAdjoint *adj);
  * (z), then s = 1 / (1 + exp(-(a z + b))).  Output float64 [x][y][z] in
  * host memory.  Noise, smoothing and the logistic run on the current
  * device (temporary buffers of 2 n doubles); mean and variance are summed
- * on the host in cell order. */
+ * on the host in cell order.  Dimensions below 1 print an error and leave
+ * s untouched, before any device call. */
 void lbmPorousField(const int dims[3], uint64_t seed, double sigma, double a, double b,
     double *s);
 

@@ -1076,6 +1076,11 @@ void lbmPorousField(const int dims[3], uint64_t seed, double sigma, double a, do
   // noise and smoothing on the device; the mean and the variance are summed
   // on the host in cell order (so the field does not depend on a device
   // reduction's order); the logistic on the device
+  if (dims[0] < 1 || dims[1] < 1 || dims[2] < 1) {
+    fprintf(stderr, "ERROR: lbmPorousField: dimensions %d x %d x %d (each must be >= 1)\n",
+        dims[0], dims[1], dims[2]);
+    return;
+  }
   const uint64_t n   = (uint64_t)dims[0] * dims[1] * dims[2];
   const size_t bytes = n * sizeof(double);
   double *u = nullptr, *t = nullptr, *kd = nullptr;

@@ -6,6 +6,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 from conftest import ROOT
 
@@ -53,6 +54,16 @@ def test_pattern_bandwidth_rejects_unsupported_shapes(planes, element_bytes, cel
     lib = lbmhip.Library("dp")
     with pytest.raises(ValueError):
         lib.pattern_bandwidth(planes, element_bytes, True, cells, reps)
+
+
+@pytest.mark.parametrize("dims", [(0, 4, 4), (4, -1, 4), (4, 4, 0)])
+def test_porous_field_rejects_empty_dimensions(dims):
+    """the dimension check runs before any device call and leaves s alone"""
+    lib = lbmhip.Library("dp")
+    s = np.full(8, 7.0)
+    lib.lib.lbmPorousField(lbmhip.I3(*dims), 1, 2.0, 4.0, -2.0,
+                           s.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    assert (s == 7.0).all()
 
 
 def test_registry_names_and_order():
