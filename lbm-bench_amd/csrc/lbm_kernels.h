@@ -42,6 +42,8 @@ struct StepLaunch {
   // launch shape: XCD-aware block order, non-temporal stores (bit 0) /
   // loads (bit 1), wavefronts per workgroup (4 or 8)
   int remap, nt, wavesPerBlock;
+  int band;    // AoS row order: bands of `band` y-rows with x swept inside each (0: y fastest)
+  int wgPerCu; // AoS: resident workgroups per CU, limited through the LDS allocation (0: none)
   int rowSweep; // push / AA odd SoA with nZ % 64 == 0: one wavefront per row, aligned stores
   int dither;    // FP16/BF16 storage: stochastic rounding in the step
   // rounding seeds: step number = *seqBase + seq (seqBase in device memory,

@@ -138,7 +138,7 @@ __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const
 // records, which no step reads), lanes past nZ move nothing.
 template <int V, int NT, typename S>
 __global__ __launch_bounds__(256) void kStepAoS(const S *a, S *b, const MaskWin *__restrict__ mask,
-    Params<S> p, int remap)
+    Params<S> p, int remap, int band)
 {
   using R  = typename Store<S>::R;
   using St = Store<S>;
@@ -150,14 +150,25 @@ __global__ __launch_bounds__(256) void kStepAoS(const S *a, S *b, const MaskWin 
   const int w        = (int)(threadIdx.x >> 6);
   const int wv       = __builtin_amdgcn_readfirstlane((int)(blk * 4u + (unsigned)w));
   const int lane     = (int)(threadIdx.x & 63u);
-  const int row      = wv / p.segs;
-  if (row >= p.nrows)
+  const int r        = wv / p.segs; // row in launch order
+  if (r >= p.nrows)
     return; // the whole wavefront
-  const int seg    = wv - row * p.segs;
+  const int seg    = wv - r * p.segs;
   const int z0     = seg * 64;
   const int nvalid = min(64, p.nz - z0);
   const int z      = z0 + lane;
-  const int x = row / p.ny, y = row - x * p.ny;
+  int x, y;
+  if (band > 0) {
+    // band k holds y-rows [k band, k band + h) of every x, x slowest
+    const int k = r / (band * p.nx), h = min(band, p.ny - k * band);
+    const int i = r - k * band * p.nx;
+    x           = i / h;
+    y           = k * band + (i - x * h);
+  } else {
+    x = r / p.ny;
+    y = r - x * p.ny;
+  }
+  const int row = x * p.ny + y;
   const MaskWin wc  = mask[(uint64_t)row * p.segs + seg];
   const bool fluid  = lane < nvalid && !bitAt(wc.lo, lane);
   const uint64_t c0 = (uint64_t)row * p.nz + (uint64_t)z0; // first cell of the block
@@ -242,8 +253,15 @@ static void launchAoS(const StepLaunch &L, hipStream_t s)
 {
   Params<S> p           = makeParams<S>(L);
   const unsigned blocks = rowBlocks(L.g, 4);
-  hipLaunchKernelGGL((kStepAoS<V, NT, S>), dim3(blocks), dim3(256), 0, s, (const S *)L.src,
-      (S *)L.dst, L.g.mask, p, L.remap);
+  // unused dynamic LDS that leaves room for L.wgPerCu workgroups of the
+  // 160 KiB of a CU (one workgroup takes at most 64 KiB)
+  constexpr int kStage = 4 * 64 * Q * (int)sizeof(S);
+  int pad              = 0;
+  if (L.wgPerCu > 0)
+    pad = std::min(65536, 163840 / L.wgPerCu - 256) - kStage;
+  pad = std::max(pad, 0);
+  hipLaunchKernelGGL((kStepAoS<V, NT, S>), dim3(blocks), dim3(256), (unsigned)pad, s, (const S *)L.src,
+      (S *)L.dst, L.g.mask, p, L.remap, L.band);
 }
 
 // ---- row sweep: aligned scatter of the z-shifted directions ---------------------

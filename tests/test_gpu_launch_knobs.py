@@ -1,8 +1,8 @@
 """Launch shape does not change results.
 
-Every launch knob (block order across XCDs, non-temporal hints, the row
-sweep against the per-block kernel, wavefronts per workgroup, plane
-padding) picks another way to run the same arithmetic on the same slots,
+Every launch knob (block order across XCDs, row order in bands,
+workgroups per CU, non-temporal hints, the row sweep against the per-block kernel, wavefronts
+per workgroup, plane padding) picks another way to run the same arithmetic on the same slots,
 and the stochastic rounding seeds depend only on cell and step.  So each
 setting must give the default run's state bit for bit, for FP64 and for
 FP16 storage, on a sphere channel with several 64-cell blocks per row
@@ -30,6 +30,11 @@ KNOBS = [
     {"LBM_WAVES_PER_BLOCK": "8"},
     {"LBM_PLANE_PAD": "4096"},
     {"LBM_GRAPH_STEPS": "4"},
+    {"LBM_ROW_BAND": "0"},
+    {"LBM_ROW_BAND": "3"},
+    {"LBM_ROW_BAND": "16"},
+    {"LBM_AOS_WG_PER_CU": "0"},
+    {"LBM_AOS_WG_PER_CU": "1"},
 ]
 
 
