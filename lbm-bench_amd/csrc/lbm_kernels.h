@@ -44,6 +44,7 @@ struct StepLaunch {
   int remap, nt, wavesPerBlock;
   int band;    // AoS row order: bands of `band` y-rows with x swept inside each (0: y fastest)
   int wgPerCu; // AoS: resident workgroups per CU, limited through the LDS allocation (0: none)
+  void *post;  // push-aos in two passes: Q planes of nX nY nZ post-collision values (else null)
   int rowSweep; // push / AA odd SoA with nZ % 64 == 0: one wavefront per row, aligned stores
   int dither;    // FP16/BF16 storage: stochastic rounding in the step
   // rounding seeds: step number = *seqBase + seq (seqBase in device memory,

@@ -66,6 +66,7 @@ struct LbmDevice {
   uint64_t nfluid;
   int narrays;
   void *d[2];
+  void *post; // push-aos in two passes: post-collision planes, Q x ncells
   MaskWin *dmask;
   std::vector<MaskWin> hmask;
   int active;
@@ -323,6 +324,7 @@ static StepLaunch baseLaunch(const LbmDevice *D, const CaseDataType *cd)
   L.remap         = D->remap;
   L.band          = D->band;
   L.wgPerCu       = D->wgPerCu;
+  L.post          = D->post;
   L.nt            = D->nt;
   L.wavesPerBlock = D->wavesPerBlock;
   L.rowSweep      = D->rowSweep;
@@ -662,6 +664,15 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
     HIP_OK(hipMalloc(&D->d[i], bytes));
     HIP_OK(hipMemsetAsync(D->d[i], 0, bytes, D->stream));
   }
+  // push-aos runs as two passes through SoA planes of post-collision
+  // values (LBM_AOS_PUSH_PASSES 1: the single-pass scatter, no planes)
+  D->post = nullptr;
+  if (D->layout == LAYOUT_AOS && D->prop == PROP_PUSH && envInt("LBM_AOS_PUSH_PASSES", 2) == 2) {
+    const size_t pbytes = (size_t)Q * D->ncells * D->esize;
+    printf("# push-aos in two passes: %.1f MiB of post-collision planes\n",
+        (double)pbytes / 1024.0 / 1024.0);
+    HIP_OK(hipMalloc(&D->post, pbytes));
+  }
 
   // mask windows, built on the device from the flag array
   const size_t nwin = (size_t)D->nx * D->ny * D->segs;
@@ -699,6 +710,8 @@ void kernelDeinit(LatticeDescType *ld, KernelDataType **kdp)
     destroyGraphs(D);
     for (int i = 0; i < D->narrays; ++i)
       HIP_OK(hipFree(D->d[i]));
+    if (D->post)
+      HIP_OK(hipFree(D->post));
     HIP_OK(hipFree(D->dmask));
     HIP_OK(hipFree(D->dpart));
     HIP_OK(hipFree(D->dout));

@@ -2,7 +2,7 @@
 // (LBM-Bench src/kernel_{pull,push,aa}_soa.c, src/kernel_{pull,push}_aos.c,
 // src/kernel_blk_{pull,push}_soa.c).
 //
-// Three kernels share the slot rules below:
+// Five kernels share the slot rules below:
 //   kStepRow   SoA, one wavefront per 64 consecutive z of one (x, y) row
 //              (wave w -> segment w % segs of the (w / segs)-th row), 4 per
 //              workgroup; pull, push, AA even and AA odd.
@@ -11,6 +11,9 @@
 //              elements, loads) are line-aligned through DPP lane shifts.
 //   kStepAoS   AoS push and pull: the side that touches the wavefront's own
 //              64 records goes through LDS.
+//   kPushCollideAoS, kPushGatherAoS  push-aos in two passes (the default):
+//              collide into SoA planes of post-collision values, then take
+//              each target slot from there, so every access is whole lines.
 // The blocked variants run kStepRow / kRowSweep with the rows taken in
 // tiles of 2 x 2 (x, y) instead of row after row (`tiled`); every cell is
 // independent, so the results equal the unblocked step.
@@ -248,11 +251,143 @@ __global__ __launch_bounds__(256) void kStepAoS(const S *a, S *b, const MaskWin 
   }
 }
 
+// ---- push-aos in two passes: whole lines on both sides ---------------------------
+// The single-pass push scatter leaves each record line of the target rows
+// partly written by nine rows' workgroups; lines that leave L2 before all
+// nine have written reach HBM as partial writes, which it completes by
+// read-modify-write (at 256^3, 4.1x the algorithmic write bytes even in
+// band order).  The two passes move twice the bytes, all in whole lines:
+//   pass 1 (kPushCollideAoS): a block's records come in through LDS as in
+//     the push step, collide, and f*_d goes to plane d of a SoA lattice
+//     `post` at the cell itself;
+//   pass 2 (kPushGatherAoS): target slot d of fluid cell y is what the
+//     push scatter stores there, post[inv d][y] when y - c_d is solid (the
+//     bounce-back of y's own f*_{inv d}) and post[d][y - c_d] otherwise;
+//     the block's records leave through LDS as whole lines.
+// The values are the stored (rounded) ones of pass 1, moved unchanged, so
+// both forms give the same lattice bit for bit.
+template <int NT, typename S>
+__global__ __launch_bounds__(256) void kPushCollideAoS(const S *a, S *post, const MaskWin *__restrict__ mask,
+    Params<S> p)
+{
+  using R  = typename Store<S>::R;
+  using St = Store<S>;
+  constexpr bool NTS = NT & 1, NTL = NT & 2;
+  __shared__ S stage[4][64 * Q];
+
+  const int w    = (int)(threadIdx.x >> 6);
+  const int wv   = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4u + (unsigned)w));
+  const int lane = (int)(threadIdx.x & 63u);
+  const int row  = wv / p.segs;
+  if (row >= p.nrows)
+    return; // the whole wavefront
+  const int seg     = wv - row * p.segs;
+  const int z0      = seg * 64;
+  const int nvalid  = min(64, p.nz - z0);
+  const MaskWin wc  = mask[(uint64_t)row * p.segs + seg];
+  const bool fluid  = lane < nvalid && !bitAt(wc.lo, lane);
+  const uint64_t c0 = (uint64_t)row * p.nz + (uint64_t)z0;
+  const int nel     = nvalid * Q;
+  S *st             = stage[w];
+
+  const S *src = a + c0 * Q;
+  for (int e = lane; e < nel; e += 64)
+    st[e] = getS<NTL>(&src[e]);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  R f[Q];
+#pragma unroll
+  for (int d = 0; d < Q; ++d)
+    f[d] = fluid ? St::raw(st[lane * Q + d]) : R(0);
+  if (p.addForce)
+    addForce(f, p.f1, p.f2);
+  collide(f, p.trt);
+  const Dither dz = ditherOf(c0 + (uint64_t)lane, stepSeq(p), p.dither);
+  if (lane < nvalid) {
+#pragma unroll
+    for (int d = 0; d < Q; ++d)
+      putS<NTS>(&post[(uint64_t)d * p.plane + c0 + (uint64_t)lane], St::rawStep(f[d], ditherDir(dz, d)));
+  }
+}
+
+template <int NT, typename S>
+__global__ __launch_bounds__(256) void kPushGatherAoS(const S *post, S *b, const MaskWin *__restrict__ mask,
+    Params<S> p)
+{
+  constexpr bool NTS = NT & 1, NTL = NT & 2;
+  __shared__ S stage[4][64 * Q];
+
+  const int w    = (int)(threadIdx.x >> 6);
+  const int wv   = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4u + (unsigned)w));
+  const int lane = (int)(threadIdx.x & 63u);
+  const int row  = wv / p.segs;
+  if (row >= p.nrows)
+    return; // the whole wavefront
+  const int seg     = wv - row * p.segs;
+  const int z0      = seg * 64;
+  const int nvalid  = min(64, p.nz - z0);
+  const int z       = z0 + lane;
+  const int x       = row / p.ny, y = row - x * p.ny;
+  const MaskWin wc  = mask[(uint64_t)row * p.segs + seg];
+  const bool fluid  = lane < nvalid && !bitAt(wc.lo, lane);
+  const uint64_t c0 = (uint64_t)row * p.nz + (uint64_t)z0;
+  const uint64_t c  = c0 + (uint64_t)lane;
+  const int nel     = nvalid * Q;
+  S *st             = stage[w];
+
+  const int xs[3] = { wrapDec(x, p.nx), x, wrapInc(x, p.nx) };
+  const int ys[3] = { wrapDec(y, p.ny), y, wrapInc(y, p.ny) };
+  const int zw    = min(z, p.nz - 1); // lanes past nZ read a valid cell, unused
+  const int zs[3] = { wrapDec(zw, p.nz), zw, wrapInc(zw, p.nz) };
+  auto rowOf      = [&](int ox, int oy) {
+    return (uint64_t)xs[ox + 1] * (uint64_t)p.ny + (uint64_t)ys[oy + 1];
+  };
+  S v[Q];
+#pragma unroll
+  for (int d = 0; d < Q; ++d) {
+    v[d] = S{};
+    if (fluid) {
+      uint64_t i = (uint64_t)C * p.plane + c;
+      if (d != C) {
+        const int ox = -cx(d), oy = -cy(d), oz = -cz(d);
+        const RowBits rb = rowBits(mask[rowOf(ox, oy) * p.segs + seg]);
+        const bool bb    = bitAt(oz < 0 ? rb.m : (oz > 0 ? rb.p : rb.c), lane);
+        i = bb ? (uint64_t)inv(d) * p.plane + c
+               : (uint64_t)d * p.plane + rowOf(ox, oy) * p.nz + (uint64_t)zs[oz + 1];
+      }
+      v[d] = getS<NTL>(&post[i]);
+    }
+  }
+  if (lane < nvalid) {
+#pragma unroll
+    for (int d = 0; d < Q; ++d)
+      st[lane * Q + d] = v[d];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  S *dst = b + c0 * Q;
+  for (int e = lane; e < nel; e += 64)
+    putS<NTS>(&dst[e], st[e]);
+}
+
 template <int V, int NT, typename S>
 static void launchAoS(const StepLaunch &L, hipStream_t s)
 {
   Params<S> p           = makeParams<S>(L);
   const unsigned blocks = rowBlocks(L.g, 4);
+  if constexpr (V == V_PUSH) {
+    if (L.post) {
+      Params<S> q = p;
+      q.plane     = (uint64_t)L.g.nx * L.g.ny * L.g.nz; // planes of `post`
+      hipLaunchKernelGGL((kPushCollideAoS<NT, S>), dim3(blocks), dim3(256), 0, s, (const S *)L.src,
+          (S *)L.post, L.g.mask, q);
+      hipLaunchKernelGGL((kPushGatherAoS<NT, S>), dim3(blocks), dim3(256), 0, s, (const S *)L.post,
+          (S *)L.dst, L.g.mask, q);
+      return;
+    }
+  }
   // unused dynamic LDS that leaves room for L.wgPerCu workgroups of the
   // 160 KiB of a CU (one workgroup takes at most 64 KiB)
   constexpr int kStage = 4 * 64 * Q * (int)sizeof(S);

@@ -1,15 +1,17 @@
 """Launch shape does not change results.
 
-Every launch knob (block order across XCDs, row order in bands,
-workgroups per CU, non-temporal hints, the row sweep against the per-block kernel, wavefronts
-per workgroup, plane padding) picks another way to run the same arithmetic on the same slots,
-and the stochastic rounding seeds depend only on cell and step.  So each
-setting must give the default run's state bit for bit, for FP64 and for
-FP16 storage, on a sphere channel with several 64-cell blocks per row
-(walls, solids inside the rows, the z wrap) and the body force on.
-LBM_ROW_SWEEP swaps kernels (kRowSweep / kStepRow), whose compilations
-fuse multiply-adds of the collision differently: there the bound is 1e-14
-relative (FP64) or a few FP16 quanta of the stored deviation (FP16)."""
+Every launch knob (block order across XCDs, row order in bands, workgroups
+per CU, push-aos in one pass or two, non-temporal hints, the row sweep
+against the per-block kernel, wavefronts per workgroup, plane padding)
+picks another way to run the same arithmetic on the same slots, and the
+stochastic rounding seeds depend only on cell and step.  So each setting
+must give the default run's state bit for bit, for FP64 and for FP16
+storage, on a sphere channel with several 64-cell blocks per row (walls,
+solids inside the rows, the z wrap) and the body force on.
+LBM_ROW_SWEEP (kRowSweep / kStepRow) and LBM_AOS_PUSH_PASSES (kStepAoS /
+kPushCollideAoS) swap kernels, whose compilations fuse multiply-adds of
+the collision differently: there the bound is 1e-14 relative (FP64) or a
+few FP16 quanta of the stored deviation (FP16)."""
 import numpy as np
 import pytest
 
@@ -35,6 +37,7 @@ KNOBS = [
     {"LBM_ROW_BAND": "16"},
     {"LBM_AOS_WG_PER_CU": "0"},
     {"LBM_AOS_WG_PER_CU": "1"},
+    {"LBM_AOS_PUSH_PASSES": "1"},
 ]
 
 
@@ -57,7 +60,7 @@ def test_knobs_leave_results_bit_identical(gpu_lib, monkeypatch, kernel, storage
             for k, v in knob.items():
                 m.setenv(k, v)
             got = _state(gpu_lib, kernel, storage)
-        if "LBM_ROW_SWEEP" in knob:
+        if "LBM_ROW_SWEEP" in knob or "LBM_AOS_PUSH_PASSES" in knob:
             fluid = ref != 0
             if storage == "native":
                 err = float(np.abs(got - ref)[fluid].max() / np.abs(ref[fluid]).max())
