@@ -153,7 +153,7 @@ def ref_cpu_mlups(kernel, n, iters, threads):
     return float(m.group(1)) if m else None
 
 
-def study_variants(lib, lbmhip, n, steps):
+def study_variants(lib, lbmhip, n, steps, cpu=True):
     """the seven variants on an n^3 channel (walls at y, periodic x and z),
     f = w(1+eps) from seed 0, FP64: MLUPS and GB/s each, beside LBM-Bench's
     own CPU kernels on the host (OpenMP where the reference has it: push-soa
@@ -170,12 +170,12 @@ def study_variants(lib, lbmhip, n, steps):
         gbs = BYTES_PER_LUP["native"] * ml / 1e3
         L.close()
         threads = min(16, os.cpu_count() or 1) if kernel in ("push-soa", "pull-soa") else 1
-        cpu = ref_cpu_mlups(kernel, n, 40 if threads > 1 else 8, threads)
+        host = ref_cpu_mlups(kernel, n, 40 if threads > 1 else 8, threads) if cpu else None
         rows[kernel] = {"mlups": round(ml, 1), "gb_s": round(gbs, 1),
                         "frac_of_8tb": round(gbs / HBM_PEAK_GBS, 4),
-                        "cpu_reference_mlups": cpu, "cpu_threads": threads}
+                        "cpu_reference_mlups": host, "cpu_threads": threads}
         log(f"{kernel}: {ml:.0f} MLUPS, {gbs:.0f} GB/s; LBM-Bench on {threads} host "
-            f"thread(s): {cpu} MLUPS")
+            f"thread(s): {host} MLUPS")
     return {"study": "the seven variants", "workload": f"{n}^3 channel -z, omega 1.25, Lambda "
             f"3/16, body force 1e-6, f = w(1+eps), {steps} steps, FP64", "cpu_model": cpu_model(),
             "variants": rows}
@@ -322,7 +322,7 @@ def main():
               flush=True)
         return
     if args.study == "variants256":
-        print(json.dumps(study_variants(lib, lbmhip, 256, 500)), flush=True)
+        print(json.dumps(study_variants(lib, lbmhip, 256, 500, cpu=not args.no_cpu)), flush=True)
         return
     if args.study == "ceiling":
         print(json.dumps(study_ceiling(lib, args.size)), flush=True)
