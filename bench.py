@@ -175,7 +175,7 @@ def study_variants(lib, lbmhip, n, steps, cpu=True):
                         "frac_of_8tb": round(gbs / HBM_PEAK_GBS, 4),
                         "cpu_reference_mlups": host, "cpu_threads": threads}
         log(f"{kernel}: {ml:.0f} MLUPS, {gbs:.0f} GB/s; LBM-Bench on {threads} host "
-            f"thread(s): {host} MLUPS")
+            f"thread(s): {host if cpu else 'skipped'} MLUPS")
     return {"study": "the seven variants", "workload": f"{n}^3 channel -z, omega 1.25, Lambda "
             f"3/16, body force 1e-6, f = w(1+eps), {steps} steps, FP64", "cpu_model": cpu_model(),
             "variants": rows}

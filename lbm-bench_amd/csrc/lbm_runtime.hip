@@ -370,8 +370,9 @@ static void launchStep(LbmDevice *D, StepLaunch L, int variant, int srcIdx, int 
   // CU, so less of the lattice is in flight per XCD than its 4 MB L2 holds.
   // Measured, channel, FP64: pull-aos 5,338 -> 9,762 MLUPS at 256^3 (16
   // rows of 256 per band) and 8,159 at 512^3 (8 rows); push-aos 4,176 ->
-  // 5,634 at 256^3 (8 rows), 4,846 at 512^3 (4 rows).  Each fetched 8.2x
-  // (pull) and wrote 5.2x (push) the algorithmic bytes in y-fastest order.
+  // 5,634 at 256^3 (8 rows), 4,846 at 512^3 (4 rows), in one pass.  In
+  // y-fastest order pull-aos read 8.6x and push-aos wrote 5.2x the
+  // algorithmic bytes; in bands pull-aos reads 1.14x.
   if (D->band < 0)
     L.band = D->layout != LAYOUT_AOS ? 0
            : std::max(1, std::min(L.g.ny, (variant == V_PULL ? 4096 : 2048) / L.g.nz));
