@@ -1,0 +1,94 @@
+"""The bench workload at its full size: a 512^3 channel (walls at y,
+periodic x and z) with 2000 seeded spheres r ~ U[4,12], aa-soa, omega
+1.25, Lambda 3/16, body force 1e-6, f = w(1+eps) from seed 0.  The oracle
+does not finish this size in seconds, so the checks are properties that
+hold at any size:
+  - mass is conserved: the body force and half-way bounce-back are both
+    mass-neutral, so the FP64 mass drifts by round-off only;
+  - from rest one pull step adds exactly g of x momentum per fluid cell:
+    rest populations stream, bounce back and collide unchanged, and the
+    force adds 3 w_i c_ix g, which sums to g over the 19 directions (the
+    AA variant runs steps in pairs, src/kernel_aa_soa.c:36, so its first
+    run from rest already meets the walls);
+  - an AA run of 1 step runs 2, as the reference's loop does; steps split
+    into eager launches and graph replays (1 -> 2, then 32) give the
+    per-cell moments of one run of 34 steps bit for bit;
+  - FP16 storage stays within the precision form's velocity error bound of
+    1e-3 against the FP64 run.
+Device memory: 20.4 GB of FP64 PDFs (40.8 GB for the two-array pull step)
+and 5.1 GB of FP16 at most at once; host memory: two 4.3 GB arrays of
+per-cell moments."""
+import numpy as np
+import pytest
+
+from lbmhip import Lattice
+
+pytestmark = pytest.mark.gpu
+
+DIMS = (512, 512, 512)
+GEOMETRY = "spheres-2000"
+G = 1e-6
+KW = dict(omega=1.25, xforce=G, body_force=True, magic=3 / 16, seed=0)
+STEPS = 34
+
+
+def _lattice(lib, storage="native", kernel="aa-soa"):
+    return Lattice(lib, kernel, GEOMETRY, DIMS, (0, 0, 1), storage=storage, **KW)
+
+
+def test_first_step_from_rest_adds_g_per_fluid_cell(gpu_lib):
+    with _lattice(gpu_lib, kernel="pull-soa") as L:
+        L.init("rest", 0, 1.0, 0.0)
+        m0 = L.mass_momentum()
+        L.run(1)
+        m1 = L.mass_momentum()
+        nf = L.n_fluid
+    print(f"{nf} fluid cells; jx after one step {m1[1]:.15e}, g * fluid {G * nf:.15e}")
+    assert nf == 128_407_050
+    assert abs(m1[1] / (G * nf) - 1.0) < 1e-9
+    assert abs(m1[2]) < 1e-9 * G * nf and abs(m1[3]) < 1e-9 * G * nf
+    assert abs(m1[0] - m0[0]) / m0[0] < 1e-14
+
+
+@pytest.fixture(scope="module")
+def split_and_whole(gpu_lib):
+    """per-cell moments after runs of 1 (that is, 2) and 32 steps and after
+    one run of 34 steps, the mass before and after, and the FP64 lattice of
+    the whole run (kept open for the FP16 comparison)"""
+    with _lattice(gpu_lib) as A:
+        A.init("random", 0, 1.0, 1e-3)
+        m0 = A.mass_momentum()
+        A.run(1)
+        A.run(STEPS - 2)
+        ma = A.mass_momentum()
+        split = A.cell_moments()
+    B = _lattice(gpu_lib)
+    B.init("random", 0, 1.0, 1e-3)
+    B.run(STEPS)
+    whole = B.cell_moments()
+    yield m0, ma, split, whole, B
+    B.close()
+
+
+def test_mass_conserved(split_and_whole):
+    m0, ma, _, _, _ = split_and_whole
+    drift = abs(ma[0] - m0[0]) / m0[0]
+    print(f"mass {m0[0]:.17e} -> {ma[0]:.17e}, relative drift {drift:.2e}")
+    assert drift < 1e-13
+
+
+def test_split_run_equals_one_run(split_and_whole):
+    _, _, split, whole, _ = split_and_whole
+    assert np.isfinite(whole).all()
+    assert np.array_equal(split.view(np.uint64), whole.view(np.uint64))
+
+
+def test_fp16_storage_velocity_error(gpu_lib, split_and_whole):
+    ref = split_and_whole[4]
+    with _lattice(gpu_lib, "fp16") as L:
+        L.init("random", 0, 1.0, 1e-3)
+        L.run(STEPS)
+        cmp = L.compare_momentum(ref)
+    l2 = (cmp[0] / cmp[1]) ** 0.5
+    print(f"fp16 velocity L2 error after {STEPS} steps: {l2:.3e}")
+    assert l2 <= 1e-3
