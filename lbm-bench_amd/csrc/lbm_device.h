@@ -135,12 +135,26 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x)
 struct Dither {
   uint32_t mask, salt;
 };
-// per cell: on != 0 dithers, seq varies from step to step
+// per cell: on != 0 dithers, seq varies from step to step.  The salt
+// passes through an empty asm so that it is formed once per cell:
+// otherwise the compiler re-derives the hash's last shift-xor inside every
+// direction's seed.
 __device__ __forceinline__ Dither ditherOf(uint64_t cell, uint32_t seq, int on)
 {
-  const uint32_t h = mix32((uint32_t)cell ^ (uint32_t)(cell >> 32) ^ seq);
-  return on ? Dither{ 0xffffffffu, h } : Dither{ 0u, 0x80000000u };
+  uint32_t salt = on ? mix32((uint32_t)cell ^ (uint32_t)(cell >> 32) ^ seq) : 0x80000000u;
+  asm("" : "+v"(salt));
+  return Dither{ on ? 0xffffffffu : 0u, salt };
 }
+// ditherOf(cell, seq, 1) and ditherOf(cell, seq, 0) with the mask visible
+// to the compiler, for a kernel that tests the flag once and keeps a store
+// loop for each case
+__device__ __forceinline__ Dither ditherOn(uint64_t cell, uint32_t seq)
+{
+  uint32_t h = mix32((uint32_t)cell ^ (uint32_t)(cell >> 32) ^ seq);
+  asm("" : "+v"(h));
+  return Dither{ 0xffffffffu, h };
+}
+__device__ __forceinline__ Dither ditherNone() { return Dither{ 0u, 0x80000000u }; }
 // per direction: the cell's salt xor'ed with a direction constant
 __device__ __forceinline__ Dither ditherDir(Dither z, int d)
 {
@@ -197,7 +211,9 @@ struct Store<__half> {
   __device__ static __half rawStep(R x, Dither z)
   {
     const uint32_t seed = (__float_as_uint(x) & z.mask) ^ z.salt;
-    const half2v r      = __builtin_amdgcn_cvt_sr_f16_f32(half2v{ 0, 0 }, x, seed, false);
+    // the tied operand is the value's own register (its upper half is
+    // not used), so no register needs clearing first
+    const half2v r      = __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(half2v, x), x, seed, false);
     return __half(__half_raw{ r[0] });
   }
 };
@@ -212,7 +228,7 @@ struct Store<__hip_bfloat16> {
   __device__ static __hip_bfloat16 rawStep(R x, Dither z)
   {
     const uint32_t seed = (__float_as_uint(x) & z.mask) ^ z.salt;
-    const bf162v r      = __builtin_amdgcn_cvt_sr_bf16_f32(bf162v{ 0, 0 }, x, seed, false);
+    const bf162v r      = __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(bf162v, x), x, seed, false);
     return __hip_bfloat16(__hip_bfloat16_raw{ __builtin_bit_cast(unsigned short, r[0]) });
   }
 };

@@ -96,6 +96,30 @@ __device__ __forceinline__ T *slotPtr(T *a, int d, uint64_t plane, uint64_t cell
 {
   return scalarBase(a + (uint64_t)d * plane) + cell;
 }
+// element rowCell + z of SoA slot d, rowCell wavefront-uniform: the base
+// goes to SGPRs as a global-address-space pointer and the lane adds a
+// 32-bit byte offset, so the access is one global load / store with a
+// scalar base and no 64-bit address arithmetic per lane (z < nZ, so the
+// offset fits for any lattice size)
+template <typename T>
+__device__ __forceinline__ T *rowSlot(T *a, int d, uint64_t plane, uint64_t rowCell, int z)
+{
+  typedef __attribute__((address_space(1))) char gchar;
+  gchar *base = (gchar *)(a + (uint64_t)d * plane + rowCell);
+  asm("" : "+s"(base));
+  return (T *)(base + (uint32_t)z * (uint32_t)sizeof(T));
+}
+
+// the step's stores to the wavefront's own row: slot d (pull) or inv d (AA
+// even) of cell rowCell + z
+template <bool NTS, bool INV, typename S, typename R>
+__device__ __forceinline__ void storeRow(S *o, uint64_t plane, uint64_t rowCell, int z,
+    const R (&f)[Q], Dither dz)
+{
+#pragma unroll
+  for (int d = 0; d < Q; ++d)
+    putS<NTS>(rowSlot(o, INV ? inv(d) : d, plane, rowCell, z), Store<S>::rawStep(f[d], ditherDir(dz, d)));
+}
 
 // body force on the 10 directions with c_x != 0
 template <typename R>
@@ -123,7 +147,7 @@ struct RowCtx {
   int segs, ny, nz;
   int lane, row, seg, z;
   int xs[3], ys[3], zs[3];
-  uint64_t cell;
+  uint64_t cell, rowCell; // rowCell = row * nZ, wavefront-uniform
   bool fluid;
 
   // keepSolid: solid lanes stay (they store zeros into their own slots) when
@@ -178,7 +202,8 @@ struct RowCtx {
     zs[0] = wrapDec(z, nz);
     zs[1] = z;
     zs[2] = wrapInc(z, nz);
-    cell  = (uint64_t)row * nz + z;
+    rowCell = (uint64_t)row * nz;
+    cell    = rowCell + z;
     return true;
   }
   // row index of the neighbour row (x+ox, y+oy)

@@ -76,7 +76,7 @@ __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const
     for (int d = 0; d < Q; ++d) {
       const S *ptr;
       if (d == C) {
-        ptr = slotPtr(a, C, p.plane, cell);
+        ptr = rowSlot(a, C, p.plane, c.rowCell, c.z);
       } else {
         const bool bb = solidOf(-cx(d), -cy(d), -cz(d));
         if (V == V_PULL)
@@ -89,30 +89,34 @@ __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const
   } else {
 #pragma unroll
     for (int d = 0; d < Q; ++d)
-      f[d] = St::raw(getS<NTL>(slotPtr(a, d, p.plane, cell)));
+      f[d] = St::raw(getS<NTL>(rowSlot(a, d, p.plane, c.rowCell, c.z)));
   }
 
   if (p.addForce)
     addForce(f, p.f1, p.f2);
 
   collide(f, p.trt);
-  const Dither dz = ditherOf(cell, stepSeq(p), p.dither);
 
-  if constexpr (V == V_PULL) {
-#pragma unroll
-    for (int d = 0; d < Q; ++d)
-      putS<NTS>(slotPtr(b, d, p.plane, cell), St::rawStep(f[d], ditherDir(dz, d)));
-  } else if constexpr (V == V_AA_EVEN) {
-#pragma unroll
-    for (int d = 0; d < Q; ++d)
-      putS<NTS>(slotPtr(a, inv(d), p.plane, cell), St::rawStep(f[d], ditherDir(dz, d)));
+  if constexpr (V == V_PULL || V == V_AA_EVEN) {
+    // own-row stores; one uniform test of the rounding flag, so that each
+    // copy of the store loop has its seeds in their plain form (two xors
+    // per direction)
+    constexpr bool SR = std::is_same<S, __half>::value || std::is_same<S, __hip_bfloat16>::value;
+    S *o              = V == V_PULL ? b : a;
+    const uint64_t rc = c.rowCell;
+    const int zc      = c.z;
+    if (SR && p.dither)
+      storeRow<NTS, V == V_AA_EVEN>(o, p.plane, rc, zc, f, ditherOn(cell, stepSeq(p)));
+    else
+      storeRow<NTS, V == V_AA_EVEN>(o, p.plane, rc, zc, f, ditherNone());
   } else {
-    S *o = V == V_PUSH ? b : a;
+    const Dither dz = ditherOf(cell, stepSeq(p), p.dither);
+    S *o            = V == V_PUSH ? b : a;
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
       S *ptr;
       if (d == C) {
-        ptr = slotPtr(o, C, p.plane, cell);
+        ptr = rowSlot(o, C, p.plane, c.rowCell, c.z);
       } else {
         const bool bb = solidOf(cx(d), cy(d), cz(d));
         ptr = bb ? slotPtr(o, inv(d), p.plane, cell) : slotPtr(o, d, p.plane, cellOf(cx(d), cy(d), cz(d)));
