@@ -84,7 +84,8 @@ int lbmLoadState(LatticeDescType *ld, KernelDataType *kd, const char *path);
  * hipEvent time of the launches in seconds. */
 double lbmRunSteps(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd, int steps);
 
-/* bytes of device memory held by the PDF arrays of kd */
+/* bytes of device memory held by the PDF arrays of kd (with push-aos's
+ * post-collision planes when it runs in two passes) */
 size_t lbmDeviceBytes(KernelDataType *kd);
 
 /* Plain device-to-device copy of `bytes` bytes, repeated `reps` times on one

@@ -1076,7 +1076,9 @@ double lbmRunSteps(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd, in
 
 size_t lbmDeviceBytes(KernelDataType *kd)
 {
-  return (size_t)kd->Dev->narrays * kd->Dev->nelem * kd->Dev->esize;
+  const LbmDevice *D = kd->Dev;
+  const size_t post  = D->post ? (size_t)Q * D->ncells * D->esize : 0;
+  return (size_t)D->narrays * D->nelem * D->esize + post;
 }
 
 double lbmCopyBandwidth(size_t bytes, int reps)

@@ -143,6 +143,10 @@ __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const
 // non-temporal bits apply to the contiguous side.  All lanes take part in
 // the copies: solid cells collide zeros (pull writes zeros into their
 // records, which no step reads), lanes past nZ move nothing.
+// band > 0 takes the rows in bands of `band` y-rows with x swept inside
+// each band, so that the nine rows sharing a record line run close
+// together and L2 still holds the line (the launcher also limits the
+// workgroups per CU through unused LDS for the same reason).
 template <int V, int NT, typename S>
 __global__ __launch_bounds__(256) void kStepAoS(const S *a, S *b, const MaskWin *__restrict__ mask,
     Params<S> p, int remap, int band)

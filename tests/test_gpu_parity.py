@@ -329,3 +329,15 @@ def test_thousand_steps_match_oracle(gpu_lib, oracle, kernel):
         L.run(steps)
         got = L.get_nodes()
     assert rel_err(got, exp, fluid) < TOL
+
+
+def test_push_aos_post_planes_counted(gpu_lib, monkeypatch):
+    """push-aos in two passes holds Q x cells extra elements of
+    post-collision planes, and lbmDeviceBytes counts them"""
+    dims = (6, 7, 70)
+    sizes = {}
+    for passes in ("1", "2"):
+        monkeypatch.setenv("LBM_AOS_PUSH_PASSES", passes)
+        with Lattice(gpu_lib, "push-aos", "channel", dims, (0, 0, 1)) as L:
+            sizes[passes] = L.device_bytes()
+    assert sizes["2"] - sizes["1"] == 19 * 6 * 7 * 70 * 8
