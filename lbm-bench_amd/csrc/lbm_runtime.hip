@@ -666,13 +666,20 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
     HIP_OK(hipMemsetAsync(D->d[i], 0, bytes, D->stream));
   }
   // push-aos runs as two passes through SoA planes of post-collision
-  // values (LBM_AOS_PUSH_PASSES 1: the single-pass scatter, no planes)
+  // values (LBM_AOS_PUSH_PASSES 1: the single-pass scatter, no planes).
+  // Without room for the planes it falls back to the single pass.
   D->post = nullptr;
   if (D->layout == LAYOUT_AOS && D->prop == PROP_PUSH && envInt("LBM_AOS_PUSH_PASSES", 2) == 2) {
     const size_t pbytes = (size_t)Q * D->ncells * D->esize;
-    printf("# push-aos in two passes: %.1f MiB of post-collision planes\n",
-        (double)pbytes / 1024.0 / 1024.0);
-    HIP_OK(hipMalloc(&D->post, pbytes));
+    if (hipMalloc(&D->post, pbytes) == hipSuccess) {
+      printf("# push-aos in two passes: %.1f MiB of post-collision planes\n",
+          (double)pbytes / 1024.0 / 1024.0);
+    } else {
+      (void)hipGetLastError();
+      D->post = nullptr;
+      printf("# push-aos in one pass: no room for %.1f MiB of post-collision planes\n",
+          (double)pbytes / 1024.0 / 1024.0);
+    }
   }
 
   // mask windows, built on the device from the flag array
