@@ -24,6 +24,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <rocprofiler-sdk-roctx/roctx.h>
 #include <vector>
 
 #include "lbm.h"
@@ -479,6 +480,9 @@ static void runKernel(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cdI
   prepareGraphs(D, cd, base);
   const int G = D->gvalid[0] ? D->gsteps : 0;
 
+  // the reference's LIKWID region around the kernel (main.c:122-131), as a
+  // ROCm marker range: rocprofv3 --marker-trace shows it
+  roctxRangePush("PROPKERNEL");
   HIP_OK(hipEventRecord(D->ev0, D->stream));
   if (n > 0) {
     int left = n;
@@ -519,6 +523,7 @@ static void runKernel(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cdI
   }
   HIP_OK(hipEventRecord(D->ev1, D->stream));
   HIP_OK(hipEventSynchronize(D->ev1));
+  roctxRangePop();
   float ms = 0.0f;
   HIP_OK(hipEventElapsedTime(&ms, D->ev0, D->ev1));
   kd->Duration = (double)ms * 1e-3;
