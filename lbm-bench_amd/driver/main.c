@@ -1,6 +1,7 @@
 /* main.c -- the benchmark driver: the flow and output lines of LBM-Bench's
  * src/main.c:34-189, with every kernel running on the GPU through the
  * lbm.h boundary (liblbmhip).  Extra lines: device, HBM roofline, mass. */
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -121,6 +122,13 @@ int main(int argc, char *argv[])
       duration, cd.MaxIterations, ld.nFluid / 1e6, geometryType, kernelToUse);
 
   int exitCode = 0;
+  /* an unstable run (e.g. omega far outside (0, 2)) ends with a non-finite
+   * mass; the reference would print NaN and succeed */
+  if (!isfinite(m1[0])) {
+    fprintf(stderr, "ERROR: the mass is not finite after %d steps: the run is unstable "
+                    "(omega %g)\n", cd.MaxIterations, (double)cd.Omega);
+    exitCode = EXIT_FAILURE;
+  }
   if (verify) {
     printf("# VERIFICATION: deviation from analytical solution: %e\n", (double)errorNorm);
     if (errorNorm > 0.1) {

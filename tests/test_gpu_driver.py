@@ -82,3 +82,23 @@ def test_pattern_bandwidth_measures(planes, element_bytes, in_place):
     lib = lbmhip.Library("dp")
     gbs = lib.pattern_bandwidth(planes, element_bytes, in_place, 1 << 22, 3)
     assert 1.0 < gbs < 8000.0
+
+
+def test_unstable_run_fails():
+    """omega = 20 amplifies the noise of a random start by about 19 per
+    step, so 400 steps overflow; the driver reports the non-finite mass and
+    exits with EXIT_FAILURE instead of printing NaN and succeeding"""
+    r = _run("dp", "-d", "32x32x32", "-g", "channel", "-z", "-k", "aa-soa", "-i", "400", "-o",
+             "20", "-I", "random", "-s", "1")
+    assert r.returncode == 1, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "mass is not finite" in r.stderr
+
+
+def test_device_allocation_failure_exits():
+    """a two-lattice variant of 1100^3 cells needs two 202 GB arrays, more
+    than the device holds: the second hipMalloc fails and the library exits
+    with EXIT_FAILURE and a message, as the reference's allocate() does
+    (src/allocate.c:17-31).  Nothing is launched."""
+    r = _run("dp", "-d", "1100x1100x1100", "-g", "channel", "-z", "-k", "push-soa", "-i", "2")
+    assert r.returncode == 1, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "hipMalloc" in r.stderr and "ERROR" in r.stderr
