@@ -108,6 +108,11 @@ int lbmDeviceSynchronize(void);
 /* name and compute-unit count of the current device; 0 on success */
 int lbmDeviceInfo(char *name, int len, int *computeUnits, size_t *globalMem);
 
+/* makes HIP device `index` current for the calls that follow (Init,
+ * measurement entry points); 0 on success, -1 after a message on stderr
+ * when there is no such device */
+int lbmSelectDevice(int index);
+
 /* ---------------------------------------------------------------------
  * Differentiation form (no counterpart in LBM-Bench).
  *

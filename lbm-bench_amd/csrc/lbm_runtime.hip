@@ -1226,6 +1226,18 @@ double lbmPatternBandwidth(int planes, int elementBytes, int inPlace, size_t cel
 
 int lbmDeviceSynchronize(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : 1; }
 
+int lbmSelectDevice(int index)
+{
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess)
+    n = 0;
+  if (index < 0 || index >= n) {
+    fprintf(stderr, "ERROR: no HIP device %d (%d present)\n", index, n);
+    return -1;
+  }
+  return hipSetDevice(index) == hipSuccess ? 0 : -1;
+}
+
 int lbmDeviceInfo(char *name, int len, int *computeUnits, size_t *globalMem)
 {
   int dev = 0;

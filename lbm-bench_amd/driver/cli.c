@@ -133,6 +133,9 @@ void parseArguments(int argc, char **argv, int dims[3], const char **geometryTyp
     case 'S':
       cd->StorageScale = strtod(optarg, NULL);
       break;
+    case 'D':
+      ro->device = (int)strtol(optarg, NULL, 0);
+      break;
     case 'h':
       printf(HELPTEXT);
       exit(EXIT_SUCCESS);

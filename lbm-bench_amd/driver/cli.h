@@ -17,7 +17,7 @@
 
 #include "lbm.h"
 
-#define BASE_ARGS "d:g:i:o:f:k:lVt:xyzhm:BI:s:a:p:S:"
+#define BASE_ARGS "d:g:i:o:f:k:lVt:xyzhm:BI:s:a:p:S:D:"
 
 #define HELPTEXT                                                                         \
   "Usage: lbmbench [options]\n"                                                          \
@@ -37,12 +37,14 @@
   "  -a V                 Amplitude of the random initial state\n"                       \
   "  -p TYPE              PDF storage: native|fp32|fp16|bf16\n"                          \
   "  -S V                 Storage scale (power of two)\n"                                \
+  "  -D N                 GPU to run on (default: HIP's default device)\n"                \
   "  -h                   Print this help\n"
 
 typedef struct {
   int initMode; /* LBM_INIT_* */
   uint64_t seed;
   double amplitude;
+  int device; /* -D: HIP device index; -1: HIP's default device */
 } RunOptions;
 
 void parseArguments(int argc, char **argv, int dims[3], const char **geometryType,

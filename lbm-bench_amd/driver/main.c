@@ -20,7 +20,7 @@ int main(int argc, char *argv[])
   int verify               = 0;
   char *kernelToUse        = "push-soa";
   int periodic[3]          = { 0, 0, 0 };
-  RunOptions ro            = { LBM_INIT_ZERO, 0, 1e-3 };
+  RunOptions ro            = { LBM_INIT_ZERO, 0, 1e-3, -1 };
 
   CaseDataType cd;
   cd.MaxIterations     = 10;
@@ -39,6 +39,8 @@ int main(int argc, char *argv[])
   LatticeDescType ld;
   geometryCreate(geometryType, dims, periodic, &ld);
 
+  if (ro.device >= 0 && lbmSelectDevice(ro.device) != 0)
+    exit(EXIT_FAILURE);
   char dev[256] = "unknown";
   int cus       = 0;
   lbmDeviceInfo(dev, sizeof(dev), &cus, NULL);

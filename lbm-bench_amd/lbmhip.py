@@ -109,6 +109,8 @@ class Library:
         L.lbmPatternBandwidth.argtypes = [c_int, c_int, c_int, ctypes.c_size_t, c_int]
         L.lbmPatternBandwidth.restype = c_double
         L.lbmDeviceSynchronize.restype = c_int
+        L.lbmSelectDevice.argtypes = [c_int]
+        L.lbmSelectDevice.restype = c_int
         L.lbmDeviceInfo.argtypes = [c_char_p, c_int, ctypes.POINTER(c_int),
                                     ctypes.POINTER(ctypes.c_size_t)]
         L.kernelDensity.argtypes = [KDP, LDP]

@@ -172,3 +172,13 @@ def test_reference_driver_links_the_gpu_library():
     assert r.returncode == 0
     listed = [l.strip() for l in r.stdout.split("Available kernels:")[1].splitlines() if l.strip()]
     assert listed == list(lbmhip.KERNELS)
+
+
+def test_driver_rejects_a_missing_device():
+    """-D selects the HIP device; an index beyond the devices present (none
+    here, one on a one-GPU machine) is refused with a message and
+    EXIT_FAILURE before anything runs"""
+    r = subprocess.run([lbmhip.driver_path("dp"), "-d", "8x8x8", "-g", "channel", "-z", "-i", "2",
+                        "-D", "9"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1
+    assert "no HIP device 9" in r.stderr

@@ -35,7 +35,7 @@ def test_verification_runs(precision):
 
 def test_benchmark_output_format():
     r = _run("dp", "-d", "64x64x64", "-g", "channel", "-z", "-k", "aa-soa", "-i", "50", "-o",
-             "1.25", "-m", "0.1875", "-B", "-f", "1e-6", "-I", "random", "-s", "3")
+             "1.25", "-m", "0.1875", "-B", "-f", "1e-6", "-I", "random", "-s", "3", "-D", "0")
     assert r.returncode == 0, r.stdout[-2000:]
     m = re.search(r"^P:\s+([0-9.]+) MFLUP/s\s+d: ([0-9.]+) s\s+iter: 50\s+fnodes: ([0-9.]+) x1e6"
                   r"\s+geo: channel\s+kernel: aa-soa$", r.stdout, flags=re.M)
