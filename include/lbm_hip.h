@@ -75,8 +75,11 @@ void lbmCellMoments(LatticeDescType *ld, KernelDataType *kd, double *out);
  * checks magic, version, dimensions and that the obstacle cells are the
  * same, then loads the PDFs, so a run resumed from a file continues as the
  * uninterrupted run would.  Both return 0, or -1 after a message on stderr
- * (I/O error, mismatch, bad checksum; after a checksum failure the
- * lattice's PDFs are undefined). */
+ * (I/O error, mismatch, truncated or trailing data, bad checksum).
+ * lbmLoadState reads the PDF section twice: the first pass checks its
+ * length and checksum, and only a file that passes is loaded, so a failed
+ * load leaves the lattice's PDFs as they were (unless the file changes
+ * between the passes, which is reported). */
 int lbmSaveState(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd, const char *path);
 int lbmLoadState(LatticeDescType *ld, KernelDataType *kd, const char *path);
 

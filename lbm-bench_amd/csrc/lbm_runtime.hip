@@ -666,6 +666,24 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
       (unsigned long long)D->ncells, (unsigned long long)(bytes * D->narrays),
       (double)(bytes * D->narrays) / 1024.0 / 1024.0,
       D->narrays == 1 ? "single lattice" : "both lattices");
+  // Refuse a lattice that cannot fit before anything is allocated or
+  // queued: the PDF arrays, the mask windows (twice while they are built)
+  // and the flag upload (as src/allocate.c:17-31 refuses on the host).
+  {
+    size_t freeB = 0, totalB = 0;
+    HIP_OK(hipMemGetInfo(&freeB, &totalB));
+    const size_t nwin   = (size_t)D->nx * D->ny * D->segs;
+    const size_t needed = bytes * (size_t)D->narrays + 2 * nwin * sizeof(MaskWin) +
+                          D->ncells * sizeof(int);
+    if (needed > freeB) {
+      fprintf(stderr,
+          "ERROR: insufficient device memory: the lattice needs %.2f GiB, %.2f GiB of %.2f GiB "
+          "are free\n",
+          (double)needed / 1073741824.0, (double)freeB / 1073741824.0,
+          (double)totalB / 1073741824.0);
+      exit(EXIT_FAILURE);
+    }
+  }
   for (int i = 0; i < D->narrays; ++i) {
     HIP_OK(hipMalloc(&D->d[i], bytes));
     HIP_OK(hipMemsetAsync(D->d[i], 0, bytes, D->stream));
@@ -1016,12 +1034,30 @@ int lbmLoadState(LatticeDescType *ld, KernelDataType *kd, const char *path)
       return stateFail(f, path, "obstacle cells differ from the lattice's");
   uint64_t sum = fnvWords(kFnvBasis, flags.data(), flags.size());
 
+  // Pass 1 reads the PDF section and checks its length and checksum
+  // without touching the lattice; only a file that passes is loaded, in
+  // pass 2, so a truncated or corrupt file leaves the state as it was.
+  const long pdfStart    = ftell(f);
   const int planes       = slabPlanes(D);
   const size_t slabCells = (size_t)planes * D->ny * D->nz;
-  double *dbuf           = nullptr;
-  HIP_OK(hipMalloc(&dbuf, slabCells * Q * sizeof(double)));
   std::vector<double> hbuf(slabCells * Q);
+  for (size_t first = 0; first < ncells; first += slabCells) {
+    const size_t count = std::min(slabCells, ncells - first);
+    if (fread(hbuf.data(), sizeof(double), count * Q, f) != count * Q)
+      return stateFail(f, path, "truncated PDF section (the lattice is unchanged)");
+    sum = fnvWords(sum, hbuf.data(), count * Q * sizeof(double));
+  }
+  if (fgetc(f) != EOF)
+    return stateFail(f, path, "trailing bytes after the PDF section (the lattice is unchanged)");
+  if (sum != h.checksum)
+    return stateFail(f, path, "checksum mismatch (the lattice is unchanged)");
+  if (pdfStart < 0 || fseek(f, pdfStart, SEEK_SET) != 0)
+    return stateFail(f, path, "cannot seek back to the PDF section (the lattice is unchanged)");
+
+  double *dbuf = nullptr;
+  HIP_OK(hipMalloc(&dbuf, slabCells * Q * sizeof(double)));
   const AuxLaunch A = auxOf(D, currentRule(D, kd));
+  uint64_t again    = fnvWords(kFnvBasis, flags.data(), flags.size());
   int rc            = 0;
   for (size_t first = 0; first < ncells; first += slabCells) {
     const size_t count = std::min(slabCells, ncells - first);
@@ -1029,7 +1065,7 @@ int lbmLoadState(LatticeDescType *ld, KernelDataType *kd, const char *path)
       rc = -1;
       break;
     }
-    sum = fnvWords(sum, hbuf.data(), count * Q * sizeof(double));
+    again = fnvWords(again, hbuf.data(), count * Q * sizeof(double));
     HIP_OK(hipMemcpyAsync(dbuf, hbuf.data(), count * Q * sizeof(double), hipMemcpyHostToDevice,
         D->stream));
     launchScatter(A, dbuf, D->stream, first, count);
@@ -1038,12 +1074,12 @@ int lbmLoadState(LatticeDescType *ld, KernelDataType *kd, const char *path)
   }
   HIP_OK(hipFree(dbuf));
   D->mirrorValid = false;
-  if (rc)
-    return stateFail(f, path, "truncated PDF section");
   fclose(f);
-  if (sum != h.checksum) {
-    fprintf(stderr, "ERROR: state file %s: checksum mismatch (the lattice's PDFs are undefined)\n",
-        path);
+  // the file changed between the two passes: the loaded PDFs are not the
+  // checked ones
+  if (rc || again != h.checksum) {
+    fprintf(stderr, "ERROR: state file %s: changed while it was loaded (the lattice's PDFs are "
+                    "undefined)\n", path);
     return -1;
   }
   return 0;

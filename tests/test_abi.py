@@ -182,3 +182,13 @@ def test_driver_rejects_a_missing_device():
                         "-D", "9"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 1
     assert "no HIP device 9" in r.stderr
+
+
+@pytest.mark.parametrize("arg", ["abc", "-3", "1.5", "", "2x"])
+def test_driver_rejects_a_malformed_device_index(arg):
+    """-D takes a non-negative decimal integer; anything else is refused with
+    a message and EXIT_FAILURE instead of silently picking a device"""
+    r = subprocess.run([lbmhip.driver_path("dp"), "-d", "8x8x8", "-g", "channel", "-z", "-i", "2",
+                        "-D", arg], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1
+    assert "device index must be a non-negative integer" in r.stdout

@@ -2,6 +2,7 @@
 (src/test.sh:61-103) for every kernel in both builds, and a benchmark run
 whose output lines keep the reference's format (src/main.c:154-171)."""
 import json
+import math
 import os
 import re
 import subprocess
@@ -95,10 +96,14 @@ def test_unstable_run_fails():
 
 
 def test_device_allocation_failure_exits():
-    """a two-lattice variant of 1100^3 cells needs two 202 GB arrays, more
-    than the device holds: the second hipMalloc fails and the library exits
+    """a two-lattice FP64 variant sized from the device's memory so that its
+    two arrays (2 x 152 B per cell) exceed it by a tenth: Init compares the
+    bytes it needs with hipMemGetInfo before its first hipMalloc and exits
     with EXIT_FAILURE and a message, as the reference's allocate() does
-    (src/allocate.c:17-31).  Nothing is launched."""
-    r = _run("dp", "-d", "1100x1100x1100", "-g", "channel", "-z", "-k", "push-soa", "-i", "2")
+    (src/allocate.c:17-31).  No device memory is allocated and nothing is
+    launched; the host still builds the cell flags (4 B per cell)."""
+    _, _, mem = lbmhip.Library("dp").device_info()
+    n = int(math.ceil((1.1 * mem / 304.0) ** (1.0 / 3.0)))
+    r = _run("dp", "-d", f"{n}x{n}x{n}", "-g", "channel", "-z", "-k", "push-soa", "-i", "2")
     assert r.returncode == 1, r.stdout[-2000:] + r.stderr[-2000:]
-    assert "hipMalloc" in r.stderr and "ERROR" in r.stderr
+    assert "insufficient device memory" in r.stderr and "hipMalloc" not in r.stderr

@@ -86,3 +86,33 @@ def test_cell_moments(gpu_lib, storage):
     exp = np.stack([f.sum(-1), f @ CX, f @ CY, f @ CZ], axis=-1)
     assert np.abs(m[fluid] - exp[fluid]).max() < 1e-14
     assert (m[~fluid] == 0).all()
+
+
+@pytest.mark.parametrize("damage", ["bitflip", "truncate", "trailing"])
+@pytest.mark.parametrize("kernel", ["aa-soa", "push-aos"])
+def test_failed_load_leaves_the_state(gpu_lib, tmp_path, kernel, damage):
+    """a corrupt, truncated or overlong file is rejected before any PDF is
+    written: the lattice keeps the state it had"""
+    dims, path = (10, 12, 16), tmp_path / "state.lbm"
+    with Lattice(gpu_lib, kernel, "channel", dims, (0, 0, 1), **KW) as L:
+        L.init("random", 1, 1.0, 1e-2)
+        L.run(3)
+        L.save_state(path)
+    raw = bytearray(path.read_bytes())
+    if damage == "bitflip":
+        raw[-9] ^= 0x40
+    elif damage == "truncate":
+        raw = raw[:-8 * 19 * 5]
+    else:
+        raw += b"\0" * 8
+    bad = tmp_path / "bad.lbm"
+    bad.write_bytes(bytes(raw))
+    with Lattice(gpu_lib, kernel, "channel", dims, (0, 0, 1), **KW) as L:
+        L.init("random", 2, 1.0, 1e-2)
+        L.run(5)
+        before = L.get_nodes()
+        with pytest.raises(OSError):
+            L.load_state(bad)
+        assert np.array_equal(L.get_nodes(), before)
+        L.run(2)                                # and it still steps as before
+        assert np.isfinite(L.get_nodes()).all()
