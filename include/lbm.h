@@ -146,6 +146,12 @@ typedef struct CaseData {
   int Storage;         /* LBM_STORE_*                                       */
   double StorageScale; /* power of two; 0 selects the default for Storage  */
   double RhoRef;       /* reference density of the shifted deviation       */
+  int PressureBoundaries; /* 1: after every step, the Zou-He pressure inlet
+                             (x = 0, RhoIn) and outlet (x = nX-1, RhoOut) of
+                             kernelComputeBoundaryConditions on the cells
+                             flagged LAT_CELL_INLET / LAT_CELL_OUTLET; the
+                             reference defines it (src/lbm.c:676-785) but
+                             its time loop never calls it */
 } CaseDataType;
 
 struct LbmDevice; /* opaque device-side state (device arrays, mask, stream) */
@@ -197,7 +203,8 @@ static inline int latticeIndex(int dims[3], int x, int y, int z)
 #endif
 
 /* Sets CaseDataType's appended fields to the reference's behaviour:
- * Lambda = 1/12, no body force, native storage, RhoRef = 1, and marks them
+ * Lambda = 1/12, no body force, native storage, RhoRef = 1, no pressure
+ * boundaries in the time loop, and marks them
  * valid.  A driver that never calls it (LBM-Bench's own main.c) gets the
  * same behaviour, with the body force taken from LBM_BODY_FORCE. */
 void caseDataDefaults(CaseDataType *cd);
@@ -216,6 +223,11 @@ void geometryDestroy(LatticeDescType *ld);
 void kernelDeinit(LatticeDescType *ld, KernelDataType **kd);
 void kernelSetInitialDensity(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd);
 void kernelSetInitialVelocity(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd);
+/* Zou-He pressure inlet / outlet (src/lbm.c:676-785), applied once to the
+ * current state on the device: the cells of x = 0 flagged LAT_CELL_INLET
+ * (y, z in 1 .. n-2) get density RhoIn, those of x = nX-1 flagged
+ * LAT_CELL_OUTLET density RhoOut.  CaseDataType.PressureBoundaries applies
+ * the same after every step inside Kernel. */
 void kernelComputeBoundaryConditions(
     KernelDataType *kd, LatticeDescType *ld, CaseDataType *cd);
 void kernelAddBodyForce(KernelDataType *kd, LatticeDescType *ld, CaseDataType *cd);

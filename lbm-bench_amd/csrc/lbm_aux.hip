@@ -108,6 +108,109 @@ void launchForce(const AuxLaunch &A, double f1, double f2, hipStream_t s)
 #undef LAUNCH
 }
 
+// ---- Zou-He pressure inlet / outlet -------------------------------------------
+// kernelComputeBoundaryConditions (src/lbm.c:676-785) on the logical PDFs.
+// One thread per (y, z) of the x faces, y and z in 1 .. n-2 as in the
+// reference's loops; face[y*nZ + z] bit 0 marks (0, y, z) an inlet cell,
+// bit 1 marks (nX-1, y, z) an outlet cell.  The thread treats the inlet
+// cell and then the outlet cell, the reference's order, so a lattice one
+// cell long sees the same sequence.  Distinct threads touch distinct
+// logical (cell, direction) pairs, and the logical-slot map is one to one
+// on fluid cells, so they never share an element.
+// pending != 0: the state lacks the body force of the step just run (the
+// next step's load or the closing force pass adds 3 w_d c_dx g to every
+// fluid cell), so the values read get it added, as the completed state
+// holds them, and the five values written get it subtracted.
+// The arithmetic keeps the reference's grouping; contraction is off.
+template <typename S, typename Cm>
+__global__ __launch_bounds__(256) void kZouHe(S *a, Geom g, int rule, const uint8_t *__restrict__ face,
+    Cm rhoIn, Cm rhoOut, Cm f1, Cm f2, int pending, double rhoRef, float scale, float inv,
+    int dither, const uint32_t *seqBase, uint32_t seq)
+{
+#pragma clang fp contract(off)
+  using R  = typename Store<S>::R;
+  using St = Store<S>;
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (uint64_t)g.ny * g.nz)
+    return;
+  const int y = (int)(t / (uint64_t)g.nz), z = (int)(t % (uint64_t)g.nz);
+  if (y < 1 || y > g.ny - 2 || z < 1 || z > g.nz - 2)
+    return;
+  const unsigned bits = face[t];
+  if (bits == 0)
+    return;
+  const uint32_t seqv = (*seqBase + seq) * 0x9e3779b9u ^ 0x2545f491u;
+  auto force = [&](int d) -> Cm {
+    return cx(d) == 0 ? Cm(0) : (cx(d) > 0 ? (wclass(d) == 1 ? f1 : f2) : -(wclass(d) == 1 ? f1 : f2));
+  };
+  const Cm third = Cm(1.0) / Cm(3.0), fourth = Cm(1.0) / Cm(4.0), sixth = Cm(1.0) / Cm(6.0);
+  for (int side = 0; side < 2; ++side) {
+    if (!(bits & (1u << side)))
+      continue;
+    const int x         = side == 0 ? 0 : g.nx - 1;
+    const uint64_t cell = ((uint64_t)x * g.ny + y) * g.nz + z;
+    const Dither dz     = ditherOf(cell, seqv, dither);
+    auto get = [&](int d) -> Cm {
+      const uint64_t i = logicalSlot(g, rule, x, y, z, cell, d);
+      Cm v = St::scaled ? (Cm)(weightOf(d) * rhoRef) + (Cm)St::load(a[i], (R)inv)
+                        : (Cm)St::load(a[i], (R)inv);
+      if (pending)
+        v += force(d);
+      return v;
+    };
+    auto put = [&](int d, Cm v) {
+      if (pending)
+        v -= force(d);
+      const uint64_t i = logicalSlot(g, rule, x, y, z, cell, d);
+      a[i] = St::scaled ? St::storeStep((R)((double)v - weightOf(d) * rhoRef), (R)scale, ditherDir(dz, d))
+                        : St::storeStep((R)v, (R)scale, ditherDir(dz, d));
+    };
+    const Cm fC = get(C), fT = get(T), fB = get(B), fS = get(lbm::S), fN = get(N);
+    const Cm fTS = get(TS), fBS = get(BS), fTN = get(TN), fBN = get(BN);
+    if (side == 0) {
+      const Cm fSW = get(SW), fTW = get(TW), fW = get(W), fBW = get(BW), fNW = get(NW);
+      const Cm dens = rhoIn;
+      const Cm ux   = Cm(1.0) - (fC + (fT + fB + fS + fN) + (fTS + fBS + fTN + fBN) +
+                                  Cm(2.0) * (fSW + fTW + fW + fBW + fNW)) * (Cm(1.0) / rhoIn);
+      const Cm indep = sixth * dens * ux;
+      put(E, fW + third * dens * ux);
+      put(NE, fSW - fourth * (fN - fS) + indep);
+      put(SE, fNW + fourth * (fN - fS) + indep);
+      put(TE, fBW - fourth * (fT - fB) + indep);
+      put(BE, fTW + fourth * (fT - fB) + indep);
+    } else {
+      const Cm fNE = get(NE), fBE = get(BE), fE = get(E), fTE = get(TE), fSE = get(SE);
+      const Cm dens = rhoOut;
+      const Cm ux   = Cm(-1.0) + (fC + (fT + fB + fS + fN) + (fTS + fBS + fTN + fBN) +
+                                   Cm(2.0) * (fNE + fBE + fE + fTE + fSE)) * (Cm(1.0) / rhoOut);
+      const Cm indep = sixth * dens * ux;
+      put(W, fE - third * dens * ux);
+      put(SW, fNE + fourth * (fN - fS) - indep);
+      put(NW, fSE - fourth * (fN - fS) - indep);
+      put(BW, fTE + fourth * (fT - fB) - indep);
+      put(TW, fBE - fourth * (fT - fB) - indep);
+    }
+  }
+}
+
+void launchZouHe(const AuxLaunch &A, const ZouHeLaunch &Z, hipStream_t s)
+{
+  const uint64_t n      = (uint64_t)A.g.ny * A.g.nz;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+#define LAUNCH(T, Cm)                                                                      \
+  hipLaunchKernelGGL((kZouHe<T, Cm>), dim3(blocks), dim3(256), 0, s, (T *)A.pdf, A.g, A.rule,   \
+      Z.face, (Cm)Z.rhoIn, (Cm)Z.rhoOut, (Cm)Z.f1, (Cm)Z.f2, Z.pending, A.rhoRef, A.scale,    \
+      A.invScale, A.dither, Z.seqBase, Z.seq)
+  switch (A.storage) {
+  case ST_F64: LAUNCH(double, double); break;
+  case ST_F32: LAUNCH(float, float); break;
+  case ST_F32DEV: LAUNCH(f32dev, double); break;
+  case ST_F16: LAUNCH(__half, double); break;
+  default: LAUNCH(__hip_bfloat16, double); break;
+  }
+#undef LAUNCH
+}
+
 // ---- seeded initialisation -------------------------------------------------
 // splitmix64 finalizer (Steele, Lea & Flood 2014) chained over (seed, a, b)
 __device__ __forceinline__ uint64_t mix64(uint64_t x)

@@ -43,6 +43,7 @@ void caseDataDefaults(CaseDataType *cd)
   cd->Storage      = LBM_STORE_NATIVE;
   cd->StorageScale = 0.0;
   cd->RhoRef       = 1.0;
+  cd->PressureBoundaries = 0;
 }
 
 CaseDataType caseDataEffective(const CaseDataType *cd)
@@ -285,61 +286,6 @@ void kernelSetInitialVelocity(LatticeDescType *ld, KernelDataType *kd, CaseDataT
         equilibriumScaled(F(1.0), F(0.0), F(0.0), F(0.0), cd->Omega, pdfs);
         kd->SetNode(kd, x, y, z, pdfs);
       }
-}
-
-/* ------------------------------------------------------------------------ */
-/* Zou-He pressure inlet/outlet on x = 0 / x = nX-1 (src/lbm.c:676-785).     */
-/* Never called by the reference's driver; kept for API completeness.         */
-/* ------------------------------------------------------------------------ */
-void kernelComputeBoundaryConditions(KernelDataType *kd, LatticeDescType *ld, CaseDataType *cd)
-{
-  const PdfType third = F(1.0) / F(3.0), fourth = F(1.0) / F(4.0), sixth = F(1.0) / F(6.0);
-  const int nX = kd->Dims[0], nY = kd->Dims[1], nZ = kd->Dims[2];
-  PdfType p[N_D3Q19];
-  /* populations read at the inlet (all but the five entering +x ones) and
-   * at the outlet (all but the five entering -x ones) */
-  static const int inRead[14]  = { D3Q19_C, D3Q19_T, D3Q19_B, D3Q19_S, D3Q19_N, D3Q19_TS,
-    D3Q19_BS, D3Q19_TN, D3Q19_BN, D3Q19_SW, D3Q19_TW, D3Q19_W, D3Q19_BW, D3Q19_NW };
-  static const int outRead[14] = { D3Q19_C, D3Q19_T, D3Q19_B, D3Q19_S, D3Q19_N, D3Q19_TS,
-    D3Q19_BS, D3Q19_TN, D3Q19_BN, D3Q19_NE, D3Q19_BE, D3Q19_E, D3Q19_TE, D3Q19_SE };
-  for (int z = 1; z < nZ - 1; ++z)
-    for (int y = 1; y < nY - 1; ++y) {
-      if (ld->Lattice[latticeIndex(ld->Dims, 0, y, z)] == LAT_CELL_INLET) {
-        for (int k = 0; k < 14; ++k)
-          kd->BoundaryConditionsGetPdf(kd, 0, y, z, inRead[k], &p[inRead[k]]);
-        const PdfType dens = cd->RhoIn;
-        const PdfType ux   = F(1.0) - (p[D3Q19_C] + (p[D3Q19_T] + p[D3Q19_B] + p[D3Q19_S] + p[D3Q19_N]) +
-                                        (p[D3Q19_TS] + p[D3Q19_BS] + p[D3Q19_TN] + p[D3Q19_BN]) +
-                                        F(2.0) * (p[D3Q19_SW] + p[D3Q19_TW] + p[D3Q19_W] +
-                                                     p[D3Q19_BW] + p[D3Q19_NW])) /
-                                       dens;
-        const PdfType indep = sixth * dens * ux;
-        const PdfType ns = fourth * (p[D3Q19_N] - p[D3Q19_S]), tb = fourth * (p[D3Q19_T] - p[D3Q19_B]);
-        kd->BoundaryConditionsSetPdf(kd, 0, y, z, D3Q19_E, p[D3Q19_W] + third * dens * ux);
-        kd->BoundaryConditionsSetPdf(kd, 0, y, z, D3Q19_NE, p[D3Q19_SW] - ns + indep);
-        kd->BoundaryConditionsSetPdf(kd, 0, y, z, D3Q19_SE, p[D3Q19_NW] + ns + indep);
-        kd->BoundaryConditionsSetPdf(kd, 0, y, z, D3Q19_TE, p[D3Q19_BW] - tb + indep);
-        kd->BoundaryConditionsSetPdf(kd, 0, y, z, D3Q19_BE, p[D3Q19_TW] + tb + indep);
-      }
-      if (ld->Lattice[latticeIndex(ld->Dims, nX - 1, y, z)] == LAT_CELL_OUTLET) {
-        const int xo = nX - 1;
-        for (int k = 0; k < 14; ++k)
-          kd->BoundaryConditionsGetPdf(kd, xo, y, z, outRead[k], &p[outRead[k]]);
-        const PdfType dens = cd->RhoOut;
-        const PdfType ux   = F(-1.0) + (p[D3Q19_C] + (p[D3Q19_T] + p[D3Q19_B] + p[D3Q19_S] + p[D3Q19_N]) +
-                                         (p[D3Q19_TS] + p[D3Q19_BS] + p[D3Q19_TN] + p[D3Q19_BN]) +
-                                         F(2.0) * (p[D3Q19_NE] + p[D3Q19_BE] + p[D3Q19_E] +
-                                                      p[D3Q19_TE] + p[D3Q19_SE])) /
-                                        dens;
-        const PdfType indep = sixth * dens * ux;
-        const PdfType ns = fourth * (p[D3Q19_N] - p[D3Q19_S]), tb = fourth * (p[D3Q19_T] - p[D3Q19_B]);
-        kd->BoundaryConditionsSetPdf(kd, xo, y, z, D3Q19_W, p[D3Q19_E] - third * dens * ux);
-        kd->BoundaryConditionsSetPdf(kd, xo, y, z, D3Q19_SW, p[D3Q19_NE] + ns - indep);
-        kd->BoundaryConditionsSetPdf(kd, xo, y, z, D3Q19_NW, p[D3Q19_SE] - ns - indep);
-        kd->BoundaryConditionsSetPdf(kd, xo, y, z, D3Q19_BW, p[D3Q19_TE] + tb - indep);
-        kd->BoundaryConditionsSetPdf(kd, xo, y, z, D3Q19_TW, p[D3Q19_BE] - tb - indep);
-      }
-    }
 }
 
 /* ------------------------------------------------------------------------ */

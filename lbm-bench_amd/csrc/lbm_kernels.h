@@ -70,6 +70,19 @@ void launchForce(const AuxLaunch &A, double f1, double f2, hipStream_t s);
 // *counter += n, one lane: the last node of a step graph, so that each
 // replay seeds its rounding with new step numbers
 void launchBumpSeq(uint32_t *counter, uint32_t n, hipStream_t s);
+// Zou-He pressure inlet (x = 0) / outlet (x = nX-1) on the logical PDFs
+// (src/lbm.c:676-785); face[y*nZ + z]: bit 0 inlet at (0, y, z), bit 1
+// outlet at (nX-1, y, z).  pending: the state lacks the force 3 w c_x g
+// (f1 = 3 w1 g, f2 = 3 w2 g) of the step just run.  FP16/BF16 stores round
+// stochastically, seeded by *seqBase + seq.
+struct ZouHeLaunch {
+  const uint8_t *face;
+  double rhoIn, rhoOut, f1, f2;
+  int pending;
+  const uint32_t *seqBase;
+  uint32_t seq;
+};
+void launchZouHe(const AuxLaunch &A, const ZouHeLaunch &Z, hipStream_t s);
 // fill logical PDFs: mode 0 zero, 1 w_d*rho, 2 w_d*(rho + (2u-1)*amp)
 void launchInit(const AuxLaunch &A, int mode, uint64_t seed, double rho, double amp,
     hipStream_t s);

@@ -78,7 +78,7 @@ struct LbmDevice {
   hipGraphExec_t gexec[2];
   bool gvalid[2];
   int gsteps;
-  double gkey[6];
+  double gkey[9];
   double rhoRef;
   float scale, invScale;
   double *dpart, *dout;
@@ -94,6 +94,14 @@ struct LbmDevice {
   // themselves from seq, offset by 2^31 so that the two ranges stay apart.
   uint32_t *dseq;
   uint32_t seq, steps, dseqHost;
+  // Zou-He pressure boundaries (kernelComputeBoundaryConditions): one byte
+  // per (y, z) of the x faces, bit 0 inlet at x = 0, bit 1 outlet at
+  // x = nX-1, uploaded from the lattice's flags (dface, allocated on first
+  // use); zhOn: applied after every step of the current run (zh: its
+  // densities and force terms)
+  uint8_t *dface;
+  int zhOn;
+  ZouHeLaunch zh;
 };
 
 // ---- small helpers ---------------------------------------------------------
@@ -401,6 +409,25 @@ static void launchStep(LbmDevice *D, StepLaunch L, int variant, int srcIdx, int 
   }
   launchStepRow(L, s); // L.tiled orders the blocked variants' wavefronts in tiles
   HIP_OK(hipGetLastError());
+  if (D->zhOn) {
+    // the pressure boundaries on the state this step wrote, read through
+    // the slot rule that holds after it; the step's own force is pending
+    AuxLaunch A;
+    A.g        = L.g;
+    A.storage  = D->storage;
+    A.rule     = variant == V_PULL ? SR_PULL : (variant == V_AA_EVEN ? SR_AA_EVEN : SR_LOCAL);
+    A.pdf      = L.dst;
+    A.rhoRef   = D->rhoRef;
+    A.scale    = D->scale;
+    A.invScale = D->invScale;
+    A.dither   = D->dither;
+    A.seq      = 0;
+    ZouHeLaunch Z = D->zh;
+    Z.seqBase     = L.seqBase;
+    Z.seq         = L.seq;
+    launchZouHe(A, Z, s);
+    HIP_OK(hipGetLastError());
+  }
 }
 
 static int stepVariant(const LbmDevice *D) { return D->prop == PROP_PULL ? V_PULL : V_PUSH; }
@@ -418,10 +445,11 @@ static void destroyGraphs(LbmDevice *D)
 // AA -> one graph (even, odd, ...); push/pull -> one per source array.
 static void prepareGraphs(LbmDevice *D, const CaseDataType *cd, const StepLaunch &base)
 {
-  const double key[6] = { (double)cd->Omega, cd->MagicParam, (double)cd->XForce,
-    (double)cd->BodyForce, (double)D->gsteps, (double)D->storage };
+  const double key[9] = { (double)cd->Omega, cd->MagicParam, (double)cd->XForce,
+    (double)cd->BodyForce, (double)D->gsteps, (double)D->storage, (double)D->zhOn,
+    D->zhOn ? D->zh.rhoIn : 0.0, D->zhOn ? D->zh.rhoOut : 0.0 };
   bool same = D->gvalid[0];
-  for (int i = 0; i < 6 && same; ++i)
+  for (int i = 0; i < 9 && same; ++i)
     same = key[i] == D->gkey[i];
   if (same || D->gsteps < 2)
     return;
@@ -464,6 +492,49 @@ static AuxLaunch auxOf(const LbmDevice *D, int rule)
   return A;
 }
 
+// Uploads the inlet / outlet flags of the x faces (y, z in 1 .. n-2, as
+// src/lbm.c:693-694 visits them) from the lattice's flags; returns whether
+// any cell is flagged.  Reads 2 nY nZ flags, never the whole lattice.
+static bool uploadFaces(LbmDevice *D, const LatticeDescType *ld)
+{
+  const size_t n = (size_t)D->ny * D->nz;
+  std::vector<uint8_t> face(n, 0);
+  int dims[3]  = { D->nx, D->ny, D->nz };
+  bool any     = false;
+  for (int y = 1; y < D->ny - 1; ++y)
+    for (int z = 1; z < D->nz - 1; ++z) {
+      uint8_t b = 0;
+      if (ld->Lattice[latticeIndex(dims, 0, y, z)] == LAT_CELL_INLET)
+        b |= 1;
+      if (ld->Lattice[latticeIndex(dims, D->nx - 1, y, z)] == LAT_CELL_OUTLET)
+        b |= 2;
+      face[(size_t)y * D->nz + z] = b;
+      any |= b != 0;
+    }
+  if (!any)
+    return false;
+  if (!D->dface)
+    HIP_OK(hipMalloc(&D->dface, n));
+  HIP_OK(hipMemcpyAsync(D->dface, face.data(), n, hipMemcpyHostToDevice, D->stream));
+  HIP_OK(hipStreamSynchronize(D->stream));
+  return true;
+}
+
+static ZouHeLaunch zouHeOf(const LbmDevice *D, const CaseDataType *cd, const StepLaunch &base,
+    int pending)
+{
+  ZouHeLaunch Z;
+  Z.face    = D->dface;
+  Z.rhoIn   = (double)cd->RhoIn;
+  Z.rhoOut  = (double)cd->RhoOut;
+  Z.f1      = D->storage == ST_F32 ? (double)base.f1f : base.f1;
+  Z.f2      = D->storage == ST_F32 ? (double)base.f2f : base.f2;
+  Z.pending = pending;
+  Z.seqBase = D->dseq + 1;
+  Z.seq     = 0;
+  return Z;
+}
+
 static void runKernel(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cdIn)
 {
   const CaseDataType eff = caseDataEffective(cdIn);
@@ -477,6 +548,11 @@ static void runKernel(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cdI
     n = (n + 1) / 2 * 2;
   const int bf          = cd->BodyForce ? 1 : 0;
   const StepLaunch base = baseLaunch(D, cd);
+  // pressure boundaries after every step: the flags are read now, so a
+  // relabelling between runs takes effect at the next run
+  D->zhOn = cd->PressureBoundaries && uploadFaces(D, ld) ? 1 : 0;
+  if (D->zhOn)
+    D->zh = zouHeOf(D, cd, base, bf);
   prepareGraphs(D, cd, base);
   const int G = D->gvalid[0] ? D->gsteps : 0;
 
@@ -524,6 +600,7 @@ static void runKernel(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cdI
   HIP_OK(hipEventRecord(D->ev1, D->stream));
   HIP_OK(hipEventSynchronize(D->ev1));
   roctxRangePop();
+  D->zhOn = 0;
   float ms = 0.0f;
   HIP_OK(hipEventElapsedTime(&ms, D->ev0, D->ev1));
   kd->Duration = (double)ms * 1e-3;
@@ -655,6 +732,8 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
   D->seq           = 0;
   D->steps         = 0;
   D->dseqHost      = 0;
+  D->dface         = nullptr;
+  D->zhOn          = 0;
   kd->LoopBalance = 2.0 * Q * (double)D->esize;
 
   HIP_OK(hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking));
@@ -747,6 +826,8 @@ void kernelDeinit(LatticeDescType *ld, KernelDataType **kdp)
     HIP_OK(hipFree(D->dpart));
     HIP_OK(hipFree(D->dout));
     HIP_OK(hipFree(D->dseq));
+    if (D->dface)
+      HIP_OK(hipFree(D->dface));
     HIP_OK(hipEventDestroy(D->ev0));
     HIP_OK(hipEventDestroy(D->ev1));
     HIP_OK(hipStreamDestroy(D->stream));
@@ -806,6 +887,22 @@ void kernelAddBodyForce(KernelDataType *kd, LatticeDescType *ld, CaseDataType *c
   AuxLaunch A = auxOf(D, currentRule(D, kd));
   A.seq       = forceSeq(D);
   launchForce(A, base.f1, base.f2, D->stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(D->stream));
+  D->mirrorValid = false;
+}
+
+void kernelComputeBoundaryConditions(KernelDataType *kd, LatticeDescType *ld, CaseDataType *cdIn)
+{
+  const CaseDataType eff = caseDataEffective(cdIn);
+  const CaseDataType *cd = &eff;
+  LbmDevice *D = kd->Dev;
+  flushMirror(D);
+  if (!uploadFaces(D, ld))
+    return; // no inlet or outlet cell: nothing to do, as in the reference's loop
+  const StepLaunch base = baseLaunch(D, cd);
+  // between runs the state is complete (the closing force pass has run)
+  launchZouHe(auxOf(D, currentRule(D, kd)), zouHeOf(D, cd, base, 0), D->stream);
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(D->stream));
   D->mirrorValid = false;

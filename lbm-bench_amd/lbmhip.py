@@ -44,7 +44,8 @@ def _structs(pdf):
         _fields_ = [("Omega", pdf), ("RhoIn", pdf), ("RhoOut", pdf), ("XForce", pdf),
                     ("MaxIterations", c_int), ("StatisticsModulus", c_int),
                     ("Extensions", c_int), ("MagicParam", c_double), ("BodyForce", c_int), ("Storage", c_int),
-                    ("StorageScale", c_double), ("RhoRef", c_double)]
+                    ("StorageScale", c_double), ("RhoRef", c_double),
+                    ("PressureBoundaries", c_int)]
 
     class KernelData(ctypes.Structure):
         pass
@@ -177,7 +178,10 @@ class Lattice:
 
     def __init__(self, lib, kernel, geometry, dims, periodic=(0, 0, 0), omega=1.0,
                  xforce=0.0, body_force=False, magic=1.0 / 12.0, storage="native",
-                 scale=0.0, rho_ref=1.0, seed=0):
+                 scale=0.0, rho_ref=1.0, seed=0, pressure=None):
+        """pressure=(rho_in, rho_out): the Zou-He inlet / outlet after every
+        step (CaseDataType.PressureBoundaries) on the cells flagged inlet /
+        outlet (set_flags)"""
         self.lib = lib
         L = lib.lib
         self.ld = lib.LatticeDesc()
@@ -199,6 +203,9 @@ class Lattice:
         self.cd.Storage = STORAGE[storage]
         self.cd.StorageScale = scale
         self.cd.RhoRef = rho_ref
+        if pressure is not None:
+            self.cd.RhoIn, self.cd.RhoOut = pressure
+            self.cd.PressureBoundaries = 1
         names = lib.kernel_names()
         if kernel not in names:
             L.geometryDestroy(ctypes.byref(self.ld))

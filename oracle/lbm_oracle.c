@@ -224,27 +224,31 @@ static const int JZ_PLUS[5] = { T, TE, TW, TN, TS }, JZ_MINUS[5] = { B, BE, BW, 
 static const int PAIR[9][2] = { { N, S }, { E, W }, { T, B }, { NW, SE }, { NE, SW },
   { TW, BE }, { TE, BW }, { TS, BN }, { TN, BS } };
 
-static double sum5(const double *f, const int *plus, const int *minus)
+static inline double sum5(const double *f, const int *plus, const int *minus)
 {
   double s = f[plus[0]];
+#pragma GCC unroll 5
   for (int k = 1; k < 5; ++k)
     s += f[plus[k]];
+#pragma GCC unroll 5
   for (int k = 0; k < 5; ++k)
     s -= f[minus[k]];
   return s;
 }
 
-static void trt_collide(const double *f, double *o, const trt_t *p)
+static inline void trt_collide(const double *f, double *o, const trt_t *p)
 {
   double ux = sum5(f, JX_PLUS, JX_MINUS);
   double uy = sum5(f, JY_PLUS, JY_MINUS);
   double uz = sum5(f, JZ_PLUS, JZ_MINUS);
   double rho = f[RHO_ORDER[0]];
+#pragma GCC unroll 19
   for (int k = 1; k < Q; ++k)
     rho += f[RHO_ORDER[k]];
   double D = rho - (ux * ux + uy * uy + uz * uz) * 3.0 / 2.0;
 
   o[C] = f[C] - p->we * (f[C] - p->w0 * D);
+#pragma GCC unroll 9
   for (int k = 0; k < 9; ++k) {
     int i = PAIR[k][0], j = PAIR[k][1];
     double ui = CX[i] * ux + CY[i] * uy + CZ[i] * uz; /* exact: coefficients are 0, +-1 */
@@ -272,13 +276,25 @@ typedef struct {
   const int *lat;
   int per[3];
   int iteration;
+  int64_t nb[Q]; /* halo'd cell index of x + c_d minus that of x */
 } ost_t;
+
+/* element of slot d of halo'd cell `cell` (src/lbm.h:113-121) */
+static inline int64_t eidx(const ost_t *s, int64_t cell, int d)
+{
+  if (s->layout == L_SOA)
+    return d * s->gcells + cell;
+  return cell * Q + d;
+}
+
+static inline int64_t hcell(const ost_t *s, int64_t x, int64_t y, int64_t z)
+{
+  return (x * s->g[1] + y) * s->g[2] + z;
+}
 
 static inline int64_t pidx(const ost_t *s, int64_t x, int64_t y, int64_t z, int d)
 {
-  if (s->layout == L_SOA)
-    return d * s->gcells + (x * s->g[1] + y) * s->g[2] + z;
-  return ((x * s->g[1] + y) * s->g[2] + z) * Q + d;
+  return eidx(s, hcell(s, x, y, z), d);
 }
 
 static inline int obstacle(const ost_t *s, int x, int y, int z)
@@ -358,38 +374,51 @@ static void build_lists(ost_t *s)
 /* node accessors: which stored slot holds logical f_d of cell (x,y,z)
  * (src/lbm.c:193-294).  AA: even Iteration -> slot inv d of x - c_d,
  * otherwise (odd, or -1 before the first step) the local slot d. */
-static int64_t node_slot(const ost_t *s, int x, int y, int z, int d)
+static inline int64_t node_slot_at(const ost_t *s, int64_t cell, int d)
 {
-  int hx = x + 1, hy = y + 1, hz = z + 1;
   switch (s->prop) {
   case P_PUSH:
-    return pidx(s, hx, hy, hz, d);
+    return eidx(s, cell, d);
   case P_PULL:
-    return pidx(s, hx - CX[d], hy - CY[d], hz - CZ[d], d);
+    return eidx(s, cell - s->nb[d], d);
   default:
     if (s->iteration % 2 == 0)
-      return pidx(s, hx - CX[d], hy - CY[d], hz - CZ[d], INV[d]);
-    return pidx(s, hx, hy, hz, d);
+      return eidx(s, cell - s->nb[d], INV[d]);
+    return eidx(s, cell, d);
   }
 }
 
-/* body force after streaming, src/lbm.c:902-922 */
+static int64_t node_slot(const ost_t *s, int x, int y, int z, int d)
+{
+  return node_slot_at(s, hcell(s, x + 1, y + 1, z + 1), d);
+}
+
+/* body force after streaming, src/lbm.c:902-922: f_d += 3 w_d c_dx g on
+ * the logical PDFs of every non-obstacle cell (the slots node_slot_at
+ * names, taken as fixed offsets from the cell's first element) */
 static void add_force(ost_t *s, double *a, double xforce)
 {
   const double w[Q] = { 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 36, 1.0 / 36, 1.0 / 36,
     1.0 / 36, 1.0 / 18, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 18, 1.0 / 36, 1.0 / 36,
     1.0 / 36, 1.0 / 36, 1.0 / 3 };
   double fd[Q];
-  for (int d = 0; d < Q; ++d)
-    fd[d] = 3.0 * w[d] * CX[d] * xforce;
+  int64_t off[Q];
+  const int64_t base0 = eidx(s, 0, 0);
+  for (int d = 0; d < Q; ++d) {
+    fd[d]  = 3.0 * w[d] * CX[d] * xforce;
+    off[d] = node_slot_at(s, 0, d) - base0;
+  }
+  const int aos = s->layout == L_AOS;
 #pragma omp parallel for collapse(2) schedule(static)
   for (int x = 0; x < s->n[0]; ++x)
     for (int y = 0; y < s->n[1]; ++y)
       for (int z = 0; z < s->n[2]; ++z) {
         if (obstacle(s, x, y, z))
           continue;
+        const int64_t cell = hcell(s, x + 1, y + 1, z + 1);
+        double *b          = a + (aos ? cell * Q : cell);
         for (int d = 0; d < Q; ++d)
-          a[node_slot(s, x, y, z, d)] += fd[d];
+          b[off[d]] += fd[d];
       }
 }
 
@@ -408,22 +437,27 @@ static void copy_lists(const ost_t *s, double *a, int forward)
  * (src/kernel_push_soa.c:94-199, src/kernel_pull_soa.c:94-199); the
  * blocked variants visit the same cells in 8^3 tiles
  * (src/kernel_blk_pull_soa.c:38-47). */
-static void cell_twolattice(const ost_t *s, const double *src, double *dst, int x, int y,
-    int z, const trt_t *p)
+static inline void cell_twolattice(const ost_t *s, const double *src, double *dst, int x,
+    int y, int z, const trt_t *p)
 {
   double f[Q], o[Q];
+  const int64_t cell = hcell(s, x, y, z);
   if (s->prop == P_PULL) {
+#pragma GCC unroll 19
     for (int d = 0; d < Q; ++d)
-      f[d] = src[pidx(s, x - CX[d], y - CY[d], z - CZ[d], d)];
+      f[d] = src[eidx(s, cell - s->nb[d], d)];
     trt_collide(f, o, p);
+#pragma GCC unroll 19
     for (int d = 0; d < Q; ++d)
-      dst[pidx(s, x, y, z, d)] = o[d];
+      dst[eidx(s, cell, d)] = o[d];
   } else {
+#pragma GCC unroll 19
     for (int d = 0; d < Q; ++d)
-      f[d] = src[pidx(s, x, y, z, d)];
+      f[d] = src[eidx(s, cell, d)];
     trt_collide(f, o, p);
+#pragma GCC unroll 19
     for (int d = 0; d < Q; ++d)
-      dst[pidx(s, x + CX[d], y + CY[d], z + CZ[d], d)] = o[d];
+      dst[eidx(s, cell + s->nb[d], d)] = o[d];
   }
 }
 
@@ -466,21 +500,50 @@ static void step_aa(ost_t *s, double *a, int odd, const trt_t *p)
     for (int y = 1; y <= ny; ++y)
       for (int z = 1; z <= nz; ++z) {
         double f[Q], o[Q];
+        const int64_t cell = hcell(s, x, y, z);
         if (!odd) {
           if (obstacle(s, x - 1, y - 1, z - 1))
             continue;
           for (int d = 0; d < Q; ++d)
-            f[d] = a[pidx(s, x, y, z, d)];
+            f[d] = a[eidx(s, cell, d)];
           trt_collide(f, o, p);
           for (int d = 0; d < Q; ++d)
-            a[pidx(s, x, y, z, INV[d])] = o[d];
+            a[eidx(s, cell, INV[d])] = o[d];
         } else {
           for (int d = 0; d < Q; ++d)
-            f[d] = a[pidx(s, x - CX[d], y - CY[d], z - CZ[d], INV[d])];
+            f[d] = a[eidx(s, cell - s->nb[d], INV[d])];
           trt_collide(f, o, p);
           for (int d = 0; d < Q; ++d)
-            a[pidx(s, x + CX[d], y + CY[d], z + CZ[d], d)] = o[d];
+            a[eidx(s, cell + s->nb[d], d)] = o[d];
         }
+      }
+}
+
+static void zou_he_cell(double *f, int inlet, double rho);
+
+/* kernelComputeBoundaryConditions (src/lbm.c:676-785) on the running
+ * state, through the node accessors: the BoundaryConditionsGetPdf /
+ * SetPdf of each variant (src/lbm.c:296-396) read and write the slot that
+ * GetNode / SetNode use for that direction.  Only the five entering
+ * populations are written. */
+static void zou_he_state(const ost_t *s, double *a, double rhoIn, double rhoOut)
+{
+  static const int inSet[5] = { E, NE, SE, TE, BE }, outSet[5] = { W, SW, NW, BW, TW };
+  const int nX = s->n[0], nY = s->n[1], nZ = s->n[2];
+  double f[Q];
+  for (int z = 1; z < nZ - 1; ++z)
+    for (int y = 1; y < nY - 1; ++y)
+      for (int side = 0; side < 2; ++side) {
+        const int x = side ? nX - 1 : 0;
+        if (s->lat[lat_index(s->n, x, y, z)] != (side ? 4 : 2)) /* OUTLET : INLET */
+          continue;
+        const int64_t cell = hcell(s, x + 1, y + 1, z + 1);
+        for (int d = 0; d < Q; ++d)
+          f[d] = a[node_slot_at(s, cell, d)];
+        zou_he_cell(f, !side, side ? rhoOut : rhoIn);
+        const int *set = side ? outSet : inSet;
+        for (int k = 0; k < 5; ++k)
+          a[node_slot_at(s, cell, set[k])] = f[set[k]];
       }
 }
 
@@ -520,9 +583,10 @@ static int parse_kernel(const char *name, int *prop, int *layout, int *blocked)
  *   out         logical PDFs after the run (obstacle cells: 0)
  *   steps_done  steps actually run
  * Returns 0, or -1 for an unknown kernel. */
-int oracle_run(const char *kernel, const int dims[3], const int *lattice,
+static int run_case(const char *kernel, const int dims[3], const int *lattice,
     const int periodic[3], const double *init, int iterations, double omega, double magic,
-    double xforce, int bodyforce, double *out, int *steps_done)
+    double xforce, int bodyforce, int zouHe, double rhoIn, double rhoOut, double *out,
+    int *steps_done)
 {
   ost_t s;
   memset(&s, 0, sizeof(s));
@@ -534,6 +598,8 @@ int oracle_run(const char *kernel, const int dims[3], const int *lattice,
     s.per[a] = periodic[a];
   }
   s.gcells    = s.g[0] * s.g[1] * s.g[2];
+  for (int d = 0; d < Q; ++d)
+    s.nb[d] = (CX[d] * s.g[1] + CY[d]) * s.g[2] + CZ[d];
   s.lat       = lattice;
   s.iteration = -1;
   int narrays = s.prop == P_AA ? 1 : 2;
@@ -565,11 +631,15 @@ int oracle_run(const char *kernel, const int dims[3], const int *lattice,
       s.iteration = iter;
       if (bodyforce)
         add_force(&s, s.pdf[0], xforce);
+      if (zouHe)
+        zou_he_state(&s, s.pdf[0], rhoIn, rhoOut);
       step_aa(&s, s.pdf[0], 1, &p);
       copy_lists(&s, s.pdf[0], 1);
       s.iteration = iter + 1;
       if (bodyforce)
         add_force(&s, s.pdf[0], xforce);
+      if (zouHe)
+        zou_he_state(&s, s.pdf[0], rhoIn, rhoOut);
       done += 2;
     }
   } else {
@@ -579,6 +649,8 @@ int oracle_run(const char *kernel, const int dims[3], const int *lattice,
       copy_lists(&s, dst, 1);
       if (bodyforce)
         add_force(&s, dst, xforce);
+      if (zouHe)
+        zou_he_state(&s, dst, rhoIn, rhoOut);
       double *t = src;
       src       = dst;
       dst       = t;
@@ -601,6 +673,25 @@ int oracle_run(const char *kernel, const int dims[3], const int *lattice,
   free(s.bbSrc);
   free(s.bbDst);
   return 0;
+}
+
+int oracle_run(const char *kernel, const int dims[3], const int *lattice,
+    const int periodic[3], const double *init, int iterations, double omega, double magic,
+    double xforce, int bodyforce, double *out, int *steps_done)
+{
+  return run_case(kernel, dims, lattice, periodic, init, iterations, omega, magic, xforce,
+      bodyforce, 0, 0.0, 0.0, out, steps_done);
+}
+
+/* oracle_run with the Zou-He inlet / outlet applied after every step (after
+ * its bounce-back copies and its body force), on the cells the lattice
+ * flags LAT_CELL_INLET (x = 0) and LAT_CELL_OUTLET (x = nX-1) */
+int oracle_run_bc(const char *kernel, const int dims[3], const int *lattice,
+    const int periodic[3], const double *init, int iterations, double omega, double magic,
+    double xforce, int bodyforce, double rhoIn, double rhoOut, double *out, int *steps_done)
+{
+  return run_case(kernel, dims, lattice, periodic, init, iterations, omega, magic, xforce,
+      bodyforce, 1, rhoIn, rhoOut, out, steps_done);
 }
 
 /* --------------------------------------------------------------------- */
@@ -998,40 +1089,45 @@ void oracle_porous_field(const int dims[3], uint64_t seed, double sigma, double 
  * rhoOut by the five entering along -x.  y and z run over 1 .. n-2, as in
  * the reference, and the arithmetic keeps its grouping (rho^-1 as a
  * multiplier, the 1/4 and 1/6 factors, the sum order). */
+/* one cell's update on its 19 logical PDFs f (inlet: the five populations
+ * entering along +x; outlet: along -x) */
+static void zou_he_cell(double *f, int inlet, double rho)
+{
+  const double third = 1.0 / 3.0, fourth = 1.0 / 4.0, sixth = 1.0 / 6.0;
+  const double inv = 1.0 / rho;
+  if (inlet) {
+    const double ux  = 1.0 - (f[C] + (f[T] + f[B] + f[S] + f[N]) +
+                                 (f[TS] + f[BS] + f[TN] + f[BN]) +
+                                 2.0 * (f[SW] + f[TW] + f[W] + f[BW] + f[NW])) *
+                                inv;
+    const double ind = sixth * rho * ux;
+    f[E]  = f[W] + third * rho * ux;
+    f[NE] = f[SW] - fourth * (f[N] - f[S]) + ind;
+    f[SE] = f[NW] + fourth * (f[N] - f[S]) + ind;
+    f[TE] = f[BW] - fourth * (f[T] - f[B]) + ind;
+    f[BE] = f[TW] + fourth * (f[T] - f[B]) + ind;
+  } else {
+    const double ux  = -1.0 + (f[C] + (f[T] + f[B] + f[S] + f[N]) +
+                                  (f[TS] + f[BS] + f[TN] + f[BN]) +
+                                  2.0 * (f[NE] + f[BE] + f[E] + f[TE] + f[SE])) *
+                                 inv;
+    const double ind = sixth * rho * ux;
+    f[W]  = f[E] - third * rho * ux;
+    f[SW] = f[NE] + fourth * (f[N] - f[S]) - ind;
+    f[NW] = f[SE] - fourth * (f[N] - f[S]) - ind;
+    f[BW] = f[TE] + fourth * (f[T] - f[B]) - ind;
+    f[TW] = f[BE] - fourth * (f[T] - f[B]) - ind;
+  }
+}
+
 void oracle_zou_he(const int dims[3], const int *lat, double *pdfs, double rhoIn, double rhoOut)
 {
   const int nX = dims[0], nY = dims[1], nZ = dims[2];
-  const double inInv = 1.0 / rhoIn, outInv = 1.0 / rhoOut;
-  const double third = 1.0 / 3.0, fourth = 1.0 / 4.0, sixth = 1.0 / 6.0;
   for (int z = 1; z < nZ - 1; ++z)
     for (int y = 1; y < nY - 1; ++y) {
-      if (lat[((size_t)z * nY + y) * nX + 0] == 2) { /* LAT_CELL_INLET */
-        double *f = pdfs + (((size_t)0 * nY + y) * nZ + z) * Q;
-        const double rho = rhoIn;
-        const double ux  = 1.0 - (f[C] + (f[T] + f[B] + f[S] + f[N]) +
-                                     (f[TS] + f[BS] + f[TN] + f[BN]) +
-                                     2.0 * (f[SW] + f[TW] + f[W] + f[BW] + f[NW])) *
-                                    inInv;
-        const double ind = sixth * rho * ux;
-        f[E]  = f[W] + third * rho * ux;
-        f[NE] = f[SW] - fourth * (f[N] - f[S]) + ind;
-        f[SE] = f[NW] + fourth * (f[N] - f[S]) + ind;
-        f[TE] = f[BW] - fourth * (f[T] - f[B]) + ind;
-        f[BE] = f[TW] + fourth * (f[T] - f[B]) + ind;
-      }
-      if (lat[((size_t)z * nY + y) * nX + (nX - 1)] == 4) { /* LAT_CELL_OUTLET */
-        double *f = pdfs + (((size_t)(nX - 1) * nY + y) * nZ + z) * Q;
-        const double rho = rhoOut;
-        const double ux  = -1.0 + (f[C] + (f[T] + f[B] + f[S] + f[N]) +
-                                      (f[TS] + f[BS] + f[TN] + f[BN]) +
-                                      2.0 * (f[NE] + f[BE] + f[E] + f[TE] + f[SE])) *
-                                     outInv;
-        const double ind = sixth * rho * ux;
-        f[W]  = f[E] - third * rho * ux;
-        f[SW] = f[NE] + fourth * (f[N] - f[S]) - ind;
-        f[NW] = f[SE] - fourth * (f[N] - f[S]) - ind;
-        f[BW] = f[TE] + fourth * (f[T] - f[B]) - ind;
-        f[TW] = f[BE] - fourth * (f[T] - f[B]) - ind;
-      }
+      if (lat[((size_t)z * nY + y) * nX + 0] == 2) /* LAT_CELL_INLET */
+        zou_he_cell(pdfs + (((size_t)0 * nY + y) * nZ + z) * Q, 1, rhoIn);
+      if (lat[((size_t)z * nY + y) * nX + (nX - 1)] == 4) /* LAT_CELL_OUTLET */
+        zou_he_cell(pdfs + (((size_t)(nX - 1) * nY + y) * nZ + z) * Q, 0, rhoOut);
     }
 }

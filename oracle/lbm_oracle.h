@@ -22,6 +22,13 @@ int oracle_run(const char *kernel, const int dims[3], const int *lattice,
     const int periodic[3], const double *init, int iterations, double omega,
     double magic, double xforce, int bodyforce, double *out, int *steps_done);
 
+/* oracle_run with kernelComputeBoundaryConditions (src/lbm.c:676-785)
+ * applied after every step, after its bounce-back and body force */
+int oracle_run_bc(const char *kernel, const int dims[3], const int *lattice,
+    const int periodic[3], const double *init, int iterations, double omega,
+    double magic, double xforce, int bodyforce, double rhoIn, double rhoOut,
+    double *out, int *steps_done);
+
 /* f_i = w_i (1 + eps), eps ~ U(-amp, amp) keyed on (seed, cell, i) */
 void oracle_init_random(const int dims[3], const int *lattice, uint64_t seed,
     double amp, double rho, double *out);

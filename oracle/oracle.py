@@ -46,6 +46,10 @@ class Oracle:
                                  ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_int, _dp, _ip]
         L.oracle_run.restype = ctypes.c_int
+        L.oracle_run_bc.argtypes = [ctypes.c_char_p, _I3, _ip, _I3, _dp, ctypes.c_int,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _ip]
+        L.oracle_run_bc.restype = ctypes.c_int
         L.oracle_init_random.argtypes = [_I3, _ip, ctypes.c_uint64, ctypes.c_double,
                                          ctypes.c_double, _dp]
         L.oracle_init_random.restype = None
@@ -92,6 +96,22 @@ class Oracle:
                                  _I3(*periodic), ip, iterations, omega, magic,
                                  xforce, int(bodyforce), _ptr(out, _dp),
                                  ctypes.byref(done))
+        if rc != 0:
+            raise ValueError(f"unknown kernel {kernel}")
+        return out, done.value
+
+    def run_bc(self, kernel, dims, lattice, periodic, init, iterations, omega, magic,
+               xforce, bodyforce, rho_in, rho_out):
+        """run() with the Zou-He inlet / outlet (flags 2 / 4 of lattice) after
+        every step -> (logical pdfs, steps actually run)"""
+        out = np.zeros(tuple(dims) + (19,), dtype=np.float64)
+        lat = np.ascontiguousarray(lattice, dtype=np.int32)
+        init = np.ascontiguousarray(init, dtype=np.float64)
+        done = ctypes.c_int(0)
+        rc = self.lib.oracle_run_bc(kernel.encode(), _I3(*dims), _ptr(lat, _ip), _I3(*periodic),
+                                    _ptr(init, _dp), iterations, omega, magic, xforce,
+                                    int(bodyforce), rho_in, rho_out, _ptr(out, _dp),
+                                    ctypes.byref(done))
         if rc != 0:
             raise ValueError(f"unknown kernel {kernel}")
         return out, done.value
@@ -152,10 +172,12 @@ class Oracle:
 
 
 class Reference:
-    """LBM-Bench's own sources compiled into _ref/ (ref_harness.c)."""
+    """LBM-Bench's own sources compiled into _ref/ (ref_harness.c); omp
+    selects the OpenMP build (the reference's parallel push-soa and
+    pull-soa loops)."""
 
-    def __init__(self, sp=False):
-        name = "libref_sp.so" if sp else "libref.so"
+    def __init__(self, sp=False, omp=False):
+        name = "libref_sp.so" if sp else ("libref_omp.so" if omp else "libref.so")
         self.lib = _load(os.path.join(REF_DIR, name))
         L = self.lib
         L.ref_geometry.argtypes = [ctypes.c_char_p, _I3, _I3, _ip, _ip]
@@ -167,12 +189,17 @@ class Reference:
                                  ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_double, _dp]
         L.ref_zou_he.restype = ctypes.c_int
+        L.ref_run_bc.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _I3, _I3, _dp, ctypes.c_int,
+                                 ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_double, _dp]
+        L.ref_run_bc.restype = ctypes.c_int
         L.ref_verification_norm.argtypes = [ctypes.c_char_p]
         L.ref_verification_norm.restype = ctypes.c_double
 
     @staticmethod
-    def available(sp=False):
-        return os.path.exists(os.path.join(REF_DIR, "libref_sp.so" if sp else "libref.so"))
+    def available(sp=False, omp=False):
+        name = "libref_sp.so" if sp else ("libref_omp.so" if omp else "libref.so")
+        return os.path.exists(os.path.join(REF_DIR, name))
 
     def geometry(self, gtype, dims, periodic=(0, 0, 0)):
         nx, ny, nz = dims
@@ -209,6 +236,20 @@ class Reference:
                                  _ptr(out, _dp))
         if rc != 0:
             raise ValueError(f"unknown kernel {kernel}")
+        return out
+
+    def run_bc(self, kernel, geometry, dims, periodic, init, iterations, omega, xforce,
+               rho_in, rho_out):
+        """x faces marked inlet/outlet, then per step the kernel's one-step run
+        and the reference's kernelComputeBoundaryConditions -> logical pdfs
+        (two-lattice kernels only)"""
+        out = np.zeros(tuple(dims) + (19,), dtype=np.float64)
+        init = np.ascontiguousarray(init, dtype=np.float64)
+        rc = self.lib.ref_run_bc(kernel.encode(), geometry.encode(), _I3(*dims), _I3(*periodic),
+                                 _ptr(init, _dp), iterations, omega, xforce, rho_in, rho_out,
+                                 _ptr(out, _dp))
+        if rc != 0:
+            raise ValueError(f"ref_run_bc: kernel {kernel} (rc {rc})")
         return out
 
     def verification_norm(self, kernel):

@@ -159,6 +159,84 @@ int ref_zou_he(const char *kernel, const char *geometry, const int dims_in[3],
       rhoIn, rhoOut, out, stats);
 }
 
+/* The reference's functions composed into a time loop with pressure
+ * boundaries: the non-obstacle cells of x = 0 and x = nX-1 (y, z in
+ * 1 .. n-2) are marked inlet and outlet, init is set through SetNode, and
+ * then `iterations` times: the variant's Kernel for one step (its
+ * bounce-back copies and, under -DVERIFICATION, its body force) followed by
+ * kernelComputeBoundaryConditions (src/lbm.c:676-785).  The two-lattice
+ * kernels start every call from Pdfs[0] (src/kernel_pull_soa.c:35-36), so
+ * the harness swaps Pdfs[0] and Pdfs[1] after each one-step call.  aa-soa
+ * runs its steps in pairs (src/kernel_aa_soa.c:36) and has no one-step
+ * call: it returns -2.  Returns 0, or -1 for an unknown kernel. */
+int ref_run_bc(const char *kernel, const char *geometry, const int dims_in[3],
+    const int periodic_in[3], const double *init, int iterations, double omega, double xforce,
+    double rhoIn, double rhoOut, double *out)
+{
+  int dims[3]     = { dims_in[0], dims_in[1], dims_in[2] };
+  int periodic[3] = { periodic_in[0], periodic_in[1], periodic_in[2] };
+  KernelFunctionsType *kf = findKernel(kernel);
+  if (kf == NULL)
+    return -1;
+  if (!strcasecmp(kernel, "aa-soa"))
+    return -2;
+  CaseDataType cd;
+  cd.MaxIterations     = 1;
+  cd.RhoIn             = (PdfType)rhoIn;
+  cd.RhoOut            = (PdfType)rhoOut;
+  cd.Omega             = (PdfType)omega;
+  cd.XForce            = (PdfType)xforce;
+  cd.StatisticsModulus = 100;
+  LatticeDescType ld;
+  geometryCreate(geometry, dims, periodic, &ld);
+  for (int z = 1; z < dims[2] - 1; ++z)
+    for (int y = 1; y < dims[1] - 1; ++y) {
+      int *in   = &ld.Lattice[latticeIndex(ld.Dims, 0, y, z)];
+      int *out_ = &ld.Lattice[latticeIndex(ld.Dims, dims[0] - 1, y, z)];
+      if (*in != LAT_CELL_OBSTACLE)
+        *in = LAT_CELL_INLET;
+      if (*out_ != LAT_CELL_OBSTACLE)
+        *out_ = LAT_CELL_OUTLET;
+    }
+  KernelDataType *kd = NULL;
+  kf->Init(&ld, &kd, &cd);
+  const int nX = dims[0], nY = dims[1], nZ = dims[2];
+  PdfType pdfs[N_D3Q19];
+  for (int x = 0; x < nX; ++x)
+    for (int y = 0; y < nY; ++y)
+      for (int z = 0; z < nZ; ++z) {
+        if (ld.Lattice[latticeIndex(ld.Dims, x, y, z)] == LAT_CELL_OBSTACLE)
+          continue;
+        const double *src = init + (((size_t)x * nY + y) * nZ + z) * N_D3Q19;
+        for (int d = 0; d < N_D3Q19; ++d)
+          pdfs[d] = (PdfType)src[d];
+        kd->SetNode(kd, x, y, z, pdfs);
+      }
+  for (int it = 0; it < iterations; ++it) {
+    kd->Kernel(&ld, kd, &cd);
+    kernelComputeBoundaryConditions(kd, &ld, &cd);
+    PdfType *t  = kd->Pdfs[0];
+    kd->Pdfs[0] = kd->Pdfs[1];
+    kd->Pdfs[1] = t;
+  }
+  for (int x = 0; x < nX; ++x)
+    for (int y = 0; y < nY; ++y)
+      for (int z = 0; z < nZ; ++z) {
+        double *dst = out + (((size_t)x * nY + y) * nZ + z) * N_D3Q19;
+        if (ld.Lattice[latticeIndex(ld.Dims, x, y, z)] == LAT_CELL_OBSTACLE) {
+          for (int d = 0; d < N_D3Q19; ++d)
+            dst[d] = 0.0;
+          continue;
+        }
+        kd->GetNode(kd, x, y, z, pdfs);
+        for (int d = 0; d < N_D3Q19; ++d)
+          dst[d] = (double)pdfs[d];
+      }
+  kf->Deinit(&ld, &kd);
+  free(ld.Lattice);
+  return 0;
+}
+
 /* The reference's -V run (cli.c:84-99 + main.c:112-142): box 16^3,
  * periodic x,y, omega 1, g 1e-5, 1000 iterations, kernelSetInitial*.
  * Returns the L2 error norm printed as "VERIFICATION: deviation". */

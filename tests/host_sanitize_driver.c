@@ -1,6 +1,6 @@
 /* host_sanitize_driver.c -- runs the host C of the library
- * (lbm-bench_amd/csrc/lbm_host.c: geometry, the -V initialisers, Zou-He,
- * statistics and verify) under AddressSanitizer and
+ * (lbm-bench_amd/csrc/lbm_host.c: geometry, the -V initialisers,
+ * statistics and verify; the Zou-He boundaries run on the device) under AddressSanitizer and
  * UndefinedBehaviorSanitizer (tests/test_host_sanitizers.py).
  *
  * lbm_host.c reaches the PDFs only through KernelDataType's accessors, so
@@ -139,17 +139,6 @@ static int runCase(const char *geometry, int nx, int ny, int nz, int px, int py,
 
   kernelSetInitialDensity(&ld, &kd, &cd);
   kernelSetInitialVelocity(&ld, &kd, &cd);
-  /* inlet and outlet faces, as a Zou-He run marks them */
-  for (int z = 0; z < nz; ++z)
-    for (int y = 0; y < ny; ++y) {
-      LatticeType *in  = &ld.Lattice[latticeIndex(ld.Dims, 0, y, z)];
-      LatticeType *out = &ld.Lattice[latticeIndex(ld.Dims, nx - 1, y, z)];
-      if (*in == LAT_CELL_FLUID)
-        *in = LAT_CELL_INLET;
-      if (nx > 1 && *out == LAT_CELL_FLUID)
-        *out = LAT_CELL_OUTLET;
-    }
-  kernelComputeBoundaryConditions(&kd, &ld, &cd);
   kernelStatistics(&kd, &ld, &cd, 10);
   PdfType norm = 0;
   kernelVerify(&ld, &kd, &cd, &norm);

@@ -14,7 +14,9 @@ hold at any size:
     into eager launches and graph replays (1 -> 2, then 32) give the
     per-cell moments of one run of 34 steps bit for bit;
   - FP16 storage stays within the precision form's velocity error bound of
-    1e-3 against the FP64 run.
+    1e-3 against the FP64 run;
+  - over 5000 steps, FP32, FP16 and BF16 storage stay within stated
+    velocity-error and mass-drift bounds of the FP64 run.
 Device memory: 20.4 GB of FP64 PDFs (40.8 GB for the two-array pull step)
 and 5.1 GB of FP16 at most at once; host memory: two 4.3 GB arrays of
 per-cell moments."""
@@ -92,3 +94,42 @@ def test_fp16_storage_velocity_error(gpu_lib, split_and_whole):
     l2 = (cmp[0] / cmp[1]) ** 0.5
     print(f"fp16 velocity L2 error after {STEPS} steps: {l2:.3e}")
     assert l2 <= 1e-3
+
+
+# bounds over 5000 steps (measured on MI355X: velocity L2 error FP32 7.2e-5,
+# FP16 2.4e-4, BF16 1.4e-3; mass drift FP64 -3.5e-13, FP32 -6.7e-10, FP16
+# 4.5e-9, BF16 -2.3e-9).  FP16's 1e-3 is the precision form's target.
+HORIZON = 5000
+FORM_BOUNDS = {"fp32": (5e-4, 1e-8), "fp16": (1e-3, 1e-7), "bf16": (5e-3, 1e-7)}
+
+
+@pytest.mark.timeout(900)
+def test_storage_forms_over_5000_steps(gpu_lib):
+    """The precision form's workload at full size: the bench lattice run
+    5000 steps in FP64 and in FP32, FP16 and BF16 shifted-deviation storage
+    (FP32 collision, stochastic rounding for the 16-bit forms); each form's
+    velocity L2 error against the FP64 run and its mass drift are bounded,
+    and the FP64 mass drifts by round-off only"""
+    with _lattice(gpu_lib) as ref:
+        ref.init("random", 0, 1.0, 1e-3)
+        m0 = ref.mass_momentum()
+        for k in range(HORIZON // 1000):
+            ref.run(1000)
+            print(f"fp64: {1000 * (k + 1)} steps", flush=True)
+        m1 = ref.mass_momentum()
+        drift64 = abs(m1[0] - m0[0]) / m0[0]
+        print(f"fp64 mass drift after {HORIZON} steps {drift64:.2e}", flush=True)
+        assert np.isfinite(m1).all() and drift64 < 1e-11
+        for form, (u_bound, m_bound) in FORM_BOUNDS.items():
+            with _lattice(gpu_lib, form) as L:
+                L.init("random", 0, 1.0, 1e-3)
+                mf0 = L.mass_momentum()
+                L.run(HORIZON)
+                mf = L.mass_momentum()
+                cmp = L.compare_momentum(ref)
+            l2 = (cmp[0] / cmp[1]) ** 0.5
+            drift = abs(mf[0] - mf0[0]) / mf0[0]
+            print(f"{form}: velocity L2 error {l2:.3e}, mass drift {drift:.2e} after {HORIZON} "
+                  f"steps", flush=True)
+            assert l2 <= u_bound, form
+            assert drift <= m_bound, form

@@ -1,7 +1,8 @@
 """Parity of the GPU kernels (through the lbm.h C-ABI) with the reference's
 golden fixtures and with the oracle, plus the configurations' checks:
   64^3 channel, pull-soa, tau 0.8, Lambda 3/16, g 1e-6, 1000 steps;
-  256^3 channel, all seven variants, f_i = w_i (1 + eps), 500 steps.
+  256^3 channel, all seven variants, f_i = w_i (1 + eps), 500 steps;
+  every variant 1000 steps on two sphere channels (nZ = 128 and 100).
 FP64 tolerance: 1e-12 relative on every fluid PDF."""
 import json
 import os
@@ -223,31 +224,49 @@ def channel256(gpu_lib, oracle):
     return dims, lat, per
 
 
+@pytest.mark.timeout(900)
 def test_channel_256_all_variants(gpu_lib, oracle, channel256):
-    """256^3, the seven variants from the same seeded start, 500 steps:
-    they agree with each other, conserve mass, and the first steps agree
-    with the oracle"""
+    """256^3 channel (walls at y, periodic x and z), the seven variants from
+    the same seeded start f = w (1 + eps), eps ~ U(-1e-3, 1e-3), omega 1.25,
+    Lambda 3/16, body force 1e-6, 500 steps: every fluid PDF of every
+    variant within 1e-12 of the oracle's 500-step state, and mass conserved.
+    The oracle runs in a thread (ctypes releases the GIL) while the GPU
+    runs the variants."""
+    import threading
+    import time
     dims, lat, per = channel256
     fluid = lat.transpose(2, 1, 0) != 0
     kw = dict(omega=1.25, xforce=1e-6, body_force=True, magic=3 / 16)
     init = oracle.init_random(dims, lat, 0, 1e-3)
-    exp5, _ = oracle.run("pull-soa", dims, lat, per, init, 4, 1.25, 3 / 16, 1e-6, True)
-    base = None
-    for kernel in KERNELS:
-        with Lattice(gpu_lib, kernel, "channel", dims, (0, 0, 1), **kw) as L:
-            L.init("random", 0, 1.0, 1e-3)
-            m0 = L.mass_momentum()
-            L.run(4)
-            assert rel_err(L.get_nodes(), exp5, fluid) < TOL, kernel
-            L.run(496)
-            m1 = L.mass_momentum()
-            assert abs(m1[0] - m0[0]) / m0[0] < 1e-13, kernel
-            state = L.get_nodes()
-            if base is None:
-                base = state
-            else:
-                assert rel_err(state, base, fluid) < TOL, kernel
-            del state
+    box = {}
+
+    def run_oracle():
+        t0 = time.perf_counter()
+        box["exp"], box["done"] = oracle.run("pull-soa", dims, lat, per, init, 500, 1.25, 3 / 16,
+                                             1e-6, True)
+        box["seconds"] = time.perf_counter() - t0
+
+    th = threading.Thread(target=run_oracle)
+    th.start()
+    states = {}
+    try:
+        for kernel in KERNELS:
+            with Lattice(gpu_lib, kernel, "channel", dims, (0, 0, 1), **kw) as L:
+                L.init("random", 0, 1.0, 1e-3)
+                m0 = L.mass_momentum()
+                L.run(500)
+                m1 = L.mass_momentum()
+                assert abs(m1[0] - m0[0]) / m0[0] < 1e-13, kernel
+                states[kernel] = L.get_nodes()
+            print(f"{kernel}: 500 steps done", flush=True)
+    finally:
+        th.join()
+    assert box["done"] == 500
+    print(f"oracle: 256^3 x 500 steps in {box['seconds']:.1f} s", flush=True)
+    for kernel, state in states.items():
+        err = rel_err(state, box["exp"], fluid)
+        print(f"{kernel}: max rel err vs oracle after 500 steps {err:.3e}")
+        assert err < TOL, kernel
 
 
 def test_verification_case(gpu_lib):
@@ -313,22 +332,64 @@ def test_zou_he_matches_oracle(gpu_lib, oracle, kernel):
         assert rel_err(got, exp, fluid) < TOL, (kernel, steps)
 
 
-@pytest.mark.parametrize("kernel", ["aa-soa", "push-soa"])
-def test_thousand_steps_match_oracle(gpu_lib, oracle, kernel):
-    """1000 steps of the row-sweep kernels (two 64-blocks per row, spheres
-    and walls, the force on, Lambda 3/16) against the oracle, every PDF"""
-    dims, steps = (32, 40, 128), 1000
-    lat, per, _ = oracle.geometry("spheres-12", dims, (0, 0, 1), 9)
+@pytest.mark.parametrize("force", [False, True])
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_pressure_boundaries_every_step_match_oracle(gpu_lib, oracle, kernel, force):
+    """CaseDataType.PressureBoundaries: the Zou-He inlet / outlet
+    (src/lbm.c:676-785) applied on the device after every step, inside the
+    captured step graphs, 20 steps (one graph of 16 plus eager steps) and 7
+    more in a second run, against the oracle's time loop with the boundary
+    pass after every step; every fluid PDF within 1e-12"""
+    dims, per = (14, 11, 10), (1, 0, 1)
+    lat, per_o, _ = oracle.geometry("channel", dims, per)
+    init = oracle.init_random(dims, lat, 21, 1e-2)
+    fluid = lat.transpose(2, 1, 0) != 0
+    marked = mark_inlet_outlet(lat)
+    kw = dict(omega=1.3, xforce=1e-4, body_force=force, magic=3 / 16, pressure=(1.01, 0.99))
+    with Lattice(gpu_lib, kernel, "channel", dims, per, **kw) as L:
+        L.set_flags(marked)
+        L.set_nodes(init)
+        L.run(20)
+        got20 = L.get_nodes()
+        L.run(8)
+        got28 = L.get_nodes()
+    exp20, _ = oracle.run_bc(kernel, dims, marked, per_o, init, 20, 1.3, 3 / 16, 1e-4, force,
+                             1.01, 0.99)
+    exp28, _ = oracle.run_bc(kernel, dims, marked, per_o, exp20, 8, 1.3, 3 / 16, 1e-4, force,
+                             1.01, 0.99)
+    assert rel_err(got20, exp20, fluid) < TOL
+    assert rel_err(got28, exp28, fluid) < TOL
+    plain, _ = oracle.run(kernel, dims, lat, per_o, init, 20, 1.3, 3 / 16, 1e-4, force)
+    assert rel_err(got20, plain, fluid) > 1e-6
+
+
+# nZ a multiple of 64 (two 64-blocks per row: the row sweep's carries and
+# z wrap) and nZ = 100 (a partial last segment: kStepRow / kStepAoS lanes
+# past nZ), both with spheres cut by the walls and the periodic faces
+LONG_CASES = [("spheres-12", (32, 40, 128), 9), ("spheres-8", (20, 26, 100), 4)]
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("geo,dims,seed", LONG_CASES)
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_thousand_steps_match_oracle(gpu_lib, oracle, kernel, geo, dims, seed):
+    """1000 steps of every variant (spheres and walls, the force on,
+    Lambda 3/16) against the oracle, every fluid PDF within 1e-12
+    (src/kernel_*.c time loops)"""
+    steps = 1000
+    lat, per, _ = oracle.geometry(geo, dims, (0, 0, 1), seed)
     init = oracle.init_random(dims, lat, 5, 1e-2)
     exp, _ = oracle.run(kernel, dims, lat, per, init, steps, 1.25, 3 / 16, 1e-5, True)
     fluid = lat.transpose(2, 1, 0) != 0
-    with Lattice(gpu_lib, kernel, "spheres-12", dims, (0, 0, 1), omega=1.25, xforce=1e-5,
-                 body_force=True, magic=3 / 16, seed=9) as L:
+    with Lattice(gpu_lib, kernel, geo, dims, (0, 0, 1), omega=1.25, xforce=1e-5,
+                 body_force=True, magic=3 / 16, seed=seed) as L:
         assert (L.lattice_flags() == lat).all()
         L.set_nodes(init)
         L.run(steps)
         got = L.get_nodes()
-    assert rel_err(got, exp, fluid) < TOL
+    err = rel_err(got, exp, fluid)
+    print(f"{kernel} {geo} {dims}: max rel err after {steps} steps {err:.3e}")
+    assert err < TOL
 
 
 def test_push_aos_post_planes_counted(gpu_lib, monkeypatch):

@@ -78,3 +78,27 @@ def test_zou_he_matches_reference(oracle, ref, geo, dims, per, kernel):
         exp = ref.zou_he(kernel, geo, dims, per, init, steps, 1.25, 1e-4, 1.01, 0.99)
         assert rel_err(got, exp, fluid) < 1e-14, (kernel, steps)
         assert not np.array_equal(got, run)  # the boundary pass changed something
+
+
+TWO_LATTICE = [k for k in KERNELS if k != "aa-soa"]
+
+
+@pytest.mark.parametrize("geo,dims,per", [("channel", (12, 11, 10), (1, 0, 1)),
+                                          ("box", (9, 10, 11), (1, 0, 0))])
+@pytest.mark.parametrize("kernel", TWO_LATTICE)
+def test_per_step_zou_he_matches_reference(oracle, ref, geo, dims, per, kernel):
+    """oracle_run_bc (Zou-He after every step) against the reference's own
+    Kernel called one step at a time, each followed by its
+    kernelComputeBoundaryConditions (src/lbm.c:676-785), 20 steps with the
+    body force"""
+    lat, per_o, _ = oracle.geometry(geo, dims, per)
+    init = oracle.init_random(dims, lat, 21, 1e-2)
+    fluid = lat.transpose(2, 1, 0) != 0
+    marked = mark_inlet_outlet(lat)
+    got, done = oracle.run_bc(kernel, dims, marked, per_o, init, 20, 1.25, 1 / 12, 1e-4, True,
+                              1.01, 0.99)
+    exp = ref.run_bc(kernel, geo, dims, per, init, 20, 1.25, 1e-4, 1.01, 0.99)
+    assert done == 20
+    assert rel_err(got, exp, fluid) < 1e-14
+    plain, _ = oracle.run(kernel, dims, lat, per_o, init, 20, 1.25, 1 / 12, 1e-4, True)
+    assert rel_err(got, plain, fluid) > 1e-6  # the boundaries changed the flow
