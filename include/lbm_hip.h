@@ -51,6 +51,13 @@ void lbmCompareMomentum(LatticeDescType *ld, KernelDataType *a, KernelDataType *
  * plane in kernelVerify). */
 void lbmCellMoments(LatticeDescType *ld, KernelDataType *kd, double *out);
 
+/* x momentum of every cell of the plane x: ux[y*nZ + z] = j_x of the
+ * logical PDFs, f_E + f_NE + f_SE + f_TE + f_BE - f_W - f_NW - f_SW - f_TW
+ * - f_BW summed in that order in PdfType (the profile src/lbm.c:976-988
+ * builds through GetNode), 0 on obstacle cells.  Reduced on the device:
+ * only the plane's nY nZ values reach the host.  kernelVerify uses it. */
+void lbmPlaneVelocity(LatticeDescType *ld, KernelDataType *kd, int x, double *ux);
+
 /* On-disk state: the logical PDFs and the cell flags in one file, for
  * resume and external validation.  The reference has no state file.
  * Layout, little-endian:

@@ -3,6 +3,8 @@
 // plain copy kernel.  One thread per cell (cell = (x*nY + y)*nZ + z); they
 // run once per run or per check, not per step, so the logical-slot rule and
 // layout are runtime arguments here.
+#include <algorithm>
+
 #include "lbm_impl.h"
 
 namespace lbm {
@@ -201,6 +203,152 @@ void launchZouHe(const AuxLaunch &A, const ZouHeLaunch &Z, hipStream_t s)
   hipLaunchKernelGGL((kZouHe<T, Cm>), dim3(blocks), dim3(256), 0, s, (T *)A.pdf, A.g, A.rule,   \
       Z.face, (Cm)Z.rhoIn, (Cm)Z.rhoOut, (Cm)Z.f1, (Cm)Z.f2, Z.pending, A.rhoRef, A.scale,    \
       A.invScale, A.dither, Z.seqBase, Z.seq)
+  switch (A.storage) {
+  case ST_F64: LAUNCH(double, double); break;
+  case ST_F32: LAUNCH(float, float); break;
+  case ST_F32DEV: LAUNCH(f32dev, double); break;
+  case ST_F16: LAUNCH(__half, double); break;
+  default: LAUNCH(__hip_bfloat16, double); break;
+  }
+#undef LAUNCH
+}
+
+// ---- geometry (src/lbm.c:52-187, plus spheres) -------------------------------
+// One thread per cell in LatticeDescType order (x fastest): obstacle (0) on
+// the boundary layer of a non-periodic axis, outside the pipe's radius, or
+// inside a block; fluid (1) elsewhere.  The pipe's test is the reference's
+// expression in double, contraction off, so the cells match a host loop.
+__global__ __launch_bounds__(256) void kGeometryCells(int *__restrict__ lat, int nx, int ny,
+    int nz, int px, int py, int pz, int pipe, int blocks)
+{
+#pragma clang fp contract(off)
+  const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  if (t >= (uint64_t)nx * ny * nz)
+    return;
+  const int x = (int)(t % (uint64_t)nx);
+  const int y = (int)((t / (uint64_t)nx) % (uint64_t)ny);
+  const int z = (int)(t / ((uint64_t)nx * ny));
+  bool obst = (!px && (x == 0 || x == nx - 1)) || (!py && (y == 0 || y == ny - 1)) ||
+              (!pz && (z == 0 || z == nz - 1));
+  if (pipe) {
+    const double cz = nz / 2.0 - 0.5, cy = ny / 2.0 - 0.5;
+    const double dmin = (double)(ny < nz ? ny : nz);
+    const double r2   = (dmin / 2 - 1) * (dmin / 2 - 1);
+    if ((z - cz) * (z - cz) + (y - cy) * (y - cy) >= r2)
+      obst = true;
+  } else if (blocks > 0) {
+    const int nb = blocks;
+    if ((z % (2 * nb)) >= nb && (y % (2 * nb)) >= nb && (x % (2 * nb)) >= nb)
+      obst = true;
+  }
+  lat[t] = obst ? 0 : 1;
+}
+
+// One workgroup per sphere: its threads walk the sphere's bounding box
+// (lo..hi per axis).  A cell whose centre lies within the radius
+// (dx^2 + dy^2 + dz^2 <= r^2 in double, contraction off) becomes an
+// obstacle; a box cell beyond the lattice wraps through a periodic axis and
+// is dropped on a non-periodic one.  Overlapping spheres store the same
+// value into a shared cell.
+__global__ __launch_bounds__(256) void kCarveSpheres(int *__restrict__ lat, int nx, int ny,
+    int nz, int px, int py, int pz, const double *__restrict__ sph, const int *__restrict__ box)
+{
+#pragma clang fp contract(off)
+  const int k = blockIdx.x;
+  const double c0 = sph[4 * k], c1 = sph[4 * k + 1], c2 = sph[4 * k + 2], r = sph[4 * k + 3];
+  const int lo0 = box[6 * k], lo1 = box[6 * k + 1], lo2 = box[6 * k + 2];
+  const int64_t ex = (int64_t)box[6 * k + 3] - lo0 + 1, ey = (int64_t)box[6 * k + 4] - lo1 + 1,
+                ez = (int64_t)box[6 * k + 5] - lo2 + 1;
+  const int64_t total = ex * ey * ez;
+  const int dims[3] = { nx, ny, nz }, per[3] = { px, py, pz };
+  auto wrapAxis = [&](int v, int a, int &out) -> bool {
+    if (v >= 0 && v < dims[a]) {
+      out = v;
+      return true;
+    }
+    if (!per[a])
+      return false;
+    out = ((v % dims[a]) + dims[a]) % dims[a];
+    return true;
+  };
+  for (int64_t i = threadIdx.x; i < total; i += blockDim.x) {
+    const int x = lo0 + (int)(i % ex), y = lo1 + (int)((i / ex) % ey), z = lo2 + (int)(i / (ex * ey));
+    const double dx = x - c0, dy = y - c1, dz = z - c2;
+    if (dx * dx + dy * dy + dz * dz > r * r)
+      continue;
+    int wx, wy, wz;
+    if (!wrapAxis(z, 2, wz) || !wrapAxis(y, 1, wy) || !wrapAxis(x, 0, wx))
+      continue;
+    lat[((uint64_t)wz * ny + wy) * nx + wx] = 0;
+  }
+}
+
+// number of non-obstacle flags, one 64-bit atomic per workgroup (exact)
+__global__ __launch_bounds__(256) void kCountFluid(const int *__restrict__ lat, uint64_t n,
+    unsigned long long *count)
+{
+  __shared__ unsigned int part[4];
+  unsigned int c = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u)
+    c += lat[i] != 0;
+  for (int o = 32; o > 0; o >>= 1)
+    c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0)
+    part[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    atomicAdd(count, (unsigned long long)part[0] + part[1] + part[2] + part[3]);
+}
+
+void launchGeometry(int *lat, const int dims[3], const int per[3], int pipe, int blocks,
+    int nSpheres, const double *sph, const int *box, unsigned long long *count, hipStream_t s)
+{
+  const uint64_t n = (uint64_t)dims[0] * dims[1] * dims[2];
+  hipLaunchKernelGGL(kGeometryCells, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, lat,
+      dims[0], dims[1], dims[2], per[0], per[1], per[2], pipe, blocks);
+  if (nSpheres > 0)
+    hipLaunchKernelGGL(kCarveSpheres, dim3((unsigned)nSpheres), dim3(256), 0, s, lat, dims[0],
+        dims[1], dims[2], per[0], per[1], per[2], sph, box);
+  const uint64_t groups = std::min<uint64_t>((n + 255) / 256, 16384);
+  hipLaunchKernelGGL(kCountFluid, dim3((unsigned)std::max<uint64_t>(groups, 1)), dim3(256), 0, s,
+      lat, n, count);
+}
+
+// ---- x velocity of one x-plane (kernelVerify's profile) -----------------------
+// ux[y*nZ + z] = j_x of the logical PDFs of (x, y, z), summed in the order
+// of src/lbm.c:981-983 in the compute type Cm (PdfType for native storage:
+// what GetNode would hand the host loop); 0 on obstacle cells
+template <typename S, typename Cm>
+__global__ __launch_bounds__(256) void kPlaneUx(const S *a, Geom g, int rule, int x,
+    double rhoRef, float inv, double *ux)
+{
+  using R  = typename Store<S>::R;
+  using St = Store<S>;
+  const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  if (t >= (uint64_t)g.ny * g.nz)
+    return;
+  const int y = (int)(t / (uint64_t)g.nz), z = (int)(t % (uint64_t)g.nz);
+  if (solidAt(g.mask, g.segs, g.ny, x, y, z)) {
+    ux[t] = 0.0;
+    return;
+  }
+  const uint64_t cell = ((uint64_t)x * g.ny + y) * g.nz + z;
+  auto f = [&](int d) -> Cm {
+    const uint64_t i = logicalSlot(g, rule, x, y, z, cell, d);
+    return St::scaled ? (Cm)(weightOf(d) * rhoRef + (double)St::load(a[i], (R)inv))
+                      : (Cm)St::load(a[i], (R)inv);
+  };
+  const Cm v = f(E) + f(NE) + f(SE) + f(TE) + f(BE) - f(W) - f(NW) - f(SW) - f(TW) - f(BW);
+  ux[t] = (double)v;
+}
+
+void launchPlaneUx(const AuxLaunch &A, int x, double *ux, hipStream_t s)
+{
+  const uint64_t n      = (uint64_t)A.g.ny * A.g.nz;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+#define LAUNCH(T, Cm)                                                                      \
+  hipLaunchKernelGGL((kPlaneUx<T, Cm>), dim3(blocks), dim3(256), 0, s, (const T *)A.pdf, A.g,    \
+      A.rule, x, A.rhoRef, A.invScale, ux)
   switch (A.storage) {
   case ST_F64: LAUNCH(double, double); break;
   case ST_F32: LAUNCH(float, float); break;

@@ -1,9 +1,9 @@
-/* lbm_host.c -- host-side parts of the lbm.h boundary that do not touch
- * the GPU directly: the D3Q19 tables and kernel registry, geometry,
- * LBM-Bench's initial conditions, statistics, verification and the Zou-He
- * boundary routine.  Everything that reads PDFs goes through the kernel's
- * GetNode/SetNode (the device mirror of lbm_runtime.hip), exactly as the
- * reference's lbm.c does.
+/* lbm_host.c -- host-side parts of the lbm.h boundary: the D3Q19 tables
+ * and kernel registry, geometryCreate's type parsing and sphere
+ * parameters, LBM-Bench's initial conditions, statistics and verification.
+ * The cell flags and the verification profile are computed on the device
+ * (lbm_geometry.h, lbmPlaneVelocity); the -V initialisers write through
+ * the kernel's SetNode as the reference's lbm.c does.
  */
 #include <math.h>
 #include <stdio.h>
@@ -12,6 +12,7 @@
 #include <strings.h>
 
 #include "lbm.h"
+#include "lbm_geometry.h"
 #include "lbm_hip.h"
 
 /* velocity set and inverse directions (src/lbm.c:20-29) */
@@ -79,51 +80,21 @@ static double hashU01(uint64_t seed, uint64_t a, uint64_t b)
 }
 
 static const char *GGeoTypeStr[] = { "box", "channel", "pipe", "blocks", "fluid", "spheres" };
-enum { GEO_BOX, GEO_CHANNEL, GEO_PIPE, GEO_BLOCKS, GEO_FLUID, GEO_SPHERES };
 
 #define SPHERE_SALT 0x5350484552455321ull
 
-static void carveSpheres(LatticeType *lat, const int dims[3], const int per[3], int count,
-    uint64_t seed)
+/* centre and radius of sphere k (4 doubles) and the lowest and highest cell
+ * of its bounding box per axis (6 ints) */
+static void sphereParams(int k, const int dims[3], uint64_t seed, double *sph, int *box)
 {
-  for (int k = 0; k < count; ++k) {
-    double c[3];
-    for (int a = 0; a < 3; ++a)
-      c[a] = hashU01(seed ^ SPHERE_SALT, (uint64_t)k, (uint64_t)a) * dims[a];
-    const double r = 4.0 + 8.0 * hashU01(seed ^ SPHERE_SALT, (uint64_t)k, 3);
-    int lo[3], hi[3];
-    for (int a = 0; a < 3; ++a) {
-      lo[a] = (int)floor(c[a] - r);
-      hi[a] = (int)ceil(c[a] + r);
-    }
-    for (int z = lo[2]; z <= hi[2]; ++z) {
-      int pz = z;
-      if (pz < 0 || pz >= dims[2]) {
-        if (!per[2])
-          continue;
-        pz = ((pz % dims[2]) + dims[2]) % dims[2];
-      }
-      for (int y = lo[1]; y <= hi[1]; ++y) {
-        int py = y;
-        if (py < 0 || py >= dims[1]) {
-          if (!per[1])
-            continue;
-          py = ((py % dims[1]) + dims[1]) % dims[1];
-        }
-        for (int x = lo[0]; x <= hi[0]; ++x) {
-          const double dx = x - c[0], dy = y - c[1], dz = z - c[2];
-          if (dx * dx + dy * dy + dz * dz > r * r)
-            continue;
-          int px = x;
-          if (px < 0 || px >= dims[0]) {
-            if (!per[0])
-              continue;
-            px = ((px % dims[0]) + dims[0]) % dims[0];
-          }
-          lat[((size_t)pz * dims[1] + py) * dims[0] + px] = LAT_CELL_OBSTACLE;
-        }
-      }
-    }
+  double r;
+  for (int a = 0; a < 3; ++a)
+    sph[a] = hashU01(seed ^ SPHERE_SALT, (uint64_t)k, (uint64_t)a) * dims[a];
+  r      = 4.0 + 8.0 * hashU01(seed ^ SPHERE_SALT, (uint64_t)k, 3);
+  sph[3] = r;
+  for (int a = 0; a < 3; ++a) {
+    box[a]     = (int)floor(sph[a] - r);
+    box[3 + a] = (int)ceil(sph[a] + r);
   }
 }
 
@@ -131,23 +102,25 @@ void geometryCreate(const char *geometryType, int dims[3], int periodic[3], Latt
 {
   int type = -1, blockSize = 8, nSpheres = 0;
   if (strncasecmp("channel", geometryType, 7) == 0)
-    type = GEO_CHANNEL;
+    type = LBM_GEO_CHANNEL;
   else if (strncasecmp("box", geometryType, 3) == 0)
-    type = GEO_BOX;
+    type = LBM_GEO_BOX;
   else if (strncasecmp("pipe", geometryType, 4) == 0)
-    type = GEO_PIPE;
+    type = LBM_GEO_PIPE;
   else if (strncasecmp("blocks", geometryType, 6) == 0) {
-    type = GEO_BLOCKS;
+    type = LBM_GEO_BLOCKS;
     if (strlen(geometryType) > 7) {
       blockSize = atoi(&geometryType[7]);
       if (blockSize <= 0)
         blockSize = 8;
     }
   } else if (strncasecmp("fluid", geometryType, 5) == 0)
-    type = GEO_FLUID;
+    type = LBM_GEO_FLUID;
   else if (strncasecmp("spheres", geometryType, 7) == 0) {
-    type     = GEO_SPHERES;
+    type     = LBM_GEO_SPHERES;
     nSpheres = strlen(geometryType) > 8 ? atoi(&geometryType[8]) : 0;
+    if (nSpheres < 0)
+      nSpheres = 0;
   } else {
     printf("ERROR: unknown geometry type '%s'.\n", geometryType);
     exit(1);
@@ -160,78 +133,35 @@ void geometryCreate(const char *geometryType, int dims[3], int periodic[3], Latt
   ld->nCells          = (int)ncells;
   ld->Name            = GGeoTypeStr[type];
   LatticeType *lat    = (LatticeType *)malloc(sizeof(LatticeType) * ncells);
-  if (lat == NULL) {
+  double *sph         = (double *)malloc(sizeof(double) * 4 * (size_t)(nSpheres > 0 ? nSpheres : 1));
+  int *box            = (int *)malloc(sizeof(int) * 6 * (size_t)(nSpheres > 0 ? nSpheres : 1));
+  if (lat == NULL || sph == NULL || box == NULL) {
     fprintf(stderr, "Error: Insufficient memory to fulfill the request\n");
     exit(EXIT_FAILURE);
   }
   ld->Lattice = lat;
-  for (size_t i = 0; i < ncells; ++i)
-    lat[i] = LAT_CELL_FLUID;
 
-  if (type == GEO_CHANNEL || type == GEO_BLOCKS || type == GEO_PIPE || type == GEO_SPHERES)
+  /* periodicity forced by the type (src/lbm.c:91-102) */
+  if (type == LBM_GEO_CHANNEL || type == LBM_GEO_BLOCKS || type == LBM_GEO_PIPE ||
+      type == LBM_GEO_SPHERES)
     periodic[0] = 1;
-  else if (type == GEO_FLUID)
+  else if (type == LBM_GEO_FLUID)
     periodic[0] = periodic[1] = periodic[2] = 1;
 
-  const int nx = dims[0], ny = dims[1], nz = dims[2];
-  for (int z = 0; z < nz; ++z)
-    for (int y = 0; y < ny; ++y)
-      for (int x = 0; x < nx; ++x) {
-        const int face = (!periodic[0] && (x == 0 || x == nx - 1)) ||
-                         (!periodic[1] && (y == 0 || y == ny - 1)) ||
-                         (!periodic[2] && (z == 0 || z == nz - 1));
-        if (face)
-          lat[((size_t)z * ny + y) * nx + x] = LAT_CELL_OBSTACLE;
-      }
-
-  if (type == GEO_PIPE) {
-    const double cz = nz / 2.0 - 0.5, cy = ny / 2.0 - 0.5;
-    const double dmin = MIN(ny, nz);
-    const double r2   = (dmin / 2 - 1) * (dmin / 2 - 1);
-    for (int z = 0; z < nz; ++z)
-      for (int y = 0; y < ny; ++y)
-        if ((z - cz) * (z - cz) + (y - cy) * (y - cy) >= r2)
-          for (int x = 0; x < nx; ++x)
-            lat[((size_t)z * ny + y) * nx + x] = LAT_CELL_OBSTACLE;
-  } else if (type == GEO_BLOCKS) {
-    const int nb = blockSize;
-    for (int z = 0; z < nz; ++z) {
-      if ((z % (2 * nb)) < nb)
-        continue;
-      for (int y = 0; y < ny; ++y) {
-        if ((y % (2 * nb)) < nb)
-          continue;
-        for (int x = 0; x < nx; ++x)
-          if ((x % (2 * nb)) >= nb)
-            lat[((size_t)z * ny + y) * nx + x] = LAT_CELL_OBSTACLE;
-      }
-    }
-  } else if (type == GEO_SPHERES) {
-    carveSpheres(lat, dims, periodic, nSpheres, GGeometrySeed);
-  }
+  for (int k = 0; k < nSpheres; ++k)
+    sphereParams(k, dims, GGeometrySeed, sph + 4 * k, box + 6 * k);
+  /* the cells themselves are built on the device and copied into lat */
+  const long long nFluid = lbmBuildFlags(dims, periodic, type, blockSize, nSpheres, sph, box, lat);
+  free(sph);
+  free(box);
 
   ld->PeriodicX = periodic[0];
   ld->PeriodicY = periodic[1];
   ld->PeriodicZ = periodic[2];
-  ld->nObst = ld->nFluid = ld->nInlet = ld->nOutlet = 0;
-  for (size_t i = 0; i < ncells; ++i) {
-    switch (lat[i]) {
-    case LAT_CELL_OBSTACLE:
-      ld->nObst++;
-      break;
-    case LAT_CELL_FLUID:
-      ld->nFluid++;
-      break;
-    case LAT_CELL_INLET:
-      ld->nInlet++;
-      ld->nFluid++;
-      break;
-    case LAT_CELL_OUTLET:
-      ld->nOutlet++;
-      ld->nFluid++;
-      break;
-    }
-  }
+  ld->nFluid    = (int)nFluid;
+  ld->nObst     = (int)((long long)ncells - nFluid);
+  ld->nInlet    = 0;
+  ld->nOutlet   = 0;
 }
 
 void geometryDestroy(LatticeDescType *ld)
@@ -304,20 +234,24 @@ void kernelVerify(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cdIn, P
   const int nX = ld->Dims[0], nY = ld->Dims[1], nZ = ld->Dims[2];
   const PdfType visc = (F(1.0) / cd->Omega - F(0.5)) / F(3.0);
   PdfType *prof      = (PdfType *)calloc((size_t)nZ * nY, sizeof(PdfType));
-  PdfType pdfs[N_D3Q19];
+  double *ux         = (double *)malloc(sizeof(double) * (size_t)nZ * nY);
+  if (prof == NULL || ux == NULL) {
+    fprintf(stderr, "Error: Insufficient memory to fulfill the request\n");
+    exit(EXIT_FAILURE);
+  }
   const int xPos = nX / 2;
+  /* the plane's x momentum, summed on the device as src/lbm.c:981-983 */
+  lbmPlaneVelocity(ld, kd, xPos, ux);
   for (int z = 0; z < nZ; ++z)
     for (int y = 0; y < nY; ++y) {
       if (ld->Lattice[latticeIndex(ld->Dims, xPos, y, z)] == LAT_CELL_OBSTACLE)
         continue;
-      kd->GetNode(kd, xPos, y, z, pdfs);
-      PdfType ux = pdfs[D3Q19_E] + pdfs[D3Q19_NE] + pdfs[D3Q19_SE] + pdfs[D3Q19_TE] +
-                   pdfs[D3Q19_BE] - pdfs[D3Q19_W] - pdfs[D3Q19_NW] - pdfs[D3Q19_SW] -
-                   pdfs[D3Q19_TW] - pdfs[D3Q19_BW];
+      PdfType v = (PdfType)ux[(size_t)y * nZ + z];
       if (cd->BodyForce)
-        ux += F(0.5) * cd->XForce; /* half-step force correction */
-      prof[y * nZ + z] = ux;
+        v += F(0.5) * cd->XForce; /* half-step force correction */
+      prof[y * nZ + z] = v;
     }
+  free(ux);
   const PdfType centre = nY / F(2.0);
   const PdfType r2max  = (nY / F(2.0) - F(1.0)) * (nY / F(2.0) - F(1.0));
   const int even       = nY % 2 == 0;

@@ -83,6 +83,14 @@ struct ZouHeLaunch {
   uint32_t seq;
 };
 void launchZouHe(const AuxLaunch &A, const ZouHeLaunch &Z, hipStream_t s);
+// cell flags of geometryCreate into lat (x fastest; 0 obstacle, 1 fluid):
+// boundary layers of non-periodic axes, pipe (pipe != 0), blocks of edge
+// `blocks` (> 0), nSpheres spheres (sph: centre and radius, box: lo / hi
+// cell per axis); *count (zeroed by the caller) += fluid cells
+void launchGeometry(int *lat, const int dims[3], const int per[3], int pipe, int blocks,
+    int nSpheres, const double *sph, const int *box, unsigned long long *count, hipStream_t s);
+// ux[y*nZ + z]: x momentum of the logical PDFs of plane x (0 on obstacles)
+void launchPlaneUx(const AuxLaunch &A, int x, double *ux, hipStream_t s);
 // fill logical PDFs: mode 0 zero, 1 w_d*rho, 2 w_d*(rho + (2u-1)*amp)
 void launchInit(const AuxLaunch &A, int mode, uint64_t seed, double rho, double amp,
     hipStream_t s);

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "lbm.h"
+#include "lbm_geometry.h"
 #include "lbm_hip.h"
 #include "lbm_kernels.h"
 
@@ -633,20 +634,29 @@ static size_t storageBytes(int st)
   return st == ST_F64 ? 8 : (st == ST_F32 || st == ST_F32DEV ? 4 : 2);
 }
 
+// The kernels wrap every axis, so the boundary layers of a non-periodic
+// axis must be obstacles.  Only the faces are read (O(n^2) flags).
 static void checkGeometry(const LatticeDescType *ld)
 {
-  const int nx = ld->Dims[0], ny = ld->Dims[1], nz = ld->Dims[2];
+  const int n[3]   = { ld->Dims[0], ld->Dims[1], ld->Dims[2] };
   const int per[3] = { ld->PeriodicX, ld->PeriodicY, ld->PeriodicZ };
-  for (int z = 0; z < nz; ++z)
-    for (int y = 0; y < ny; ++y)
-      for (int x = 0; x < nx; ++x) {
-        const bool face = (!per[0] && (x == 0 || x == nx - 1)) ||
-                          (!per[1] && (y == 0 || y == ny - 1)) ||
-                          (!per[2] && (z == 0 || z == nz - 1));
-        if (face && ld->Lattice[((size_t)z * ny + y) * nx + x] != LAT_CELL_OBSTACLE)
-          fail("fluid cell on a non-periodic boundary face; the boundary layers of "
-               "non-periodic axes must be obstacles");
-      }
+  int dims[3]      = { n[0], n[1], n[2] };
+  for (int a = 0; a < 3; ++a) {
+    if (per[a])
+      continue;
+    const int b = (a + 1) % 3, c = (a + 2) % 3;
+    for (int side = 0; side < 2; ++side)
+      for (int j = 0; j < n[b]; ++j)
+        for (int k = 0; k < n[c]; ++k) {
+          int p[3];
+          p[a] = side ? n[a] - 1 : 0;
+          p[b] = j;
+          p[c] = k;
+          if (ld->Lattice[latticeIndex(dims, p[0], p[1], p[2])] != LAT_CELL_OBSTACLE)
+            fail("fluid cell on a non-periodic boundary face; the boundary layers of "
+                 "non-periodic axes must be obstacles");
+        }
+  }
 }
 
 static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataType *cdIn,
@@ -906,6 +916,61 @@ void kernelComputeBoundaryConditions(KernelDataType *kd, LatticeDescType *ld, Ca
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(D->stream));
   D->mirrorValid = false;
+}
+
+void lbmPlaneVelocity(LatticeDescType *ld, KernelDataType *kd, int x, double *ux)
+{
+  LbmDevice *D = kd->Dev;
+  if (x < 0 || x >= D->nx) {
+    fprintf(stderr, "ERROR: lbmPlaneVelocity: plane x = %d outside 0 .. %d\n", x, D->nx - 1);
+    exit(EXIT_FAILURE);
+  }
+  flushMirror(D);
+  const size_t bytes = sizeof(double) * (size_t)D->ny * D->nz;
+  double *buf        = nullptr;
+  HIP_OK(hipMalloc(&buf, bytes));
+  launchPlaneUx(auxOf(D, currentRule(D, kd)), x, buf, D->stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(ux, buf, bytes, hipMemcpyDeviceToHost, D->stream));
+  HIP_OK(hipStreamSynchronize(D->stream));
+  HIP_OK(hipFree(buf));
+}
+
+long long lbmBuildFlags(const int dims[3], const int periodic[3], int type, int blockSize,
+    int nSpheres, const double *spheres, const int *bounds, LatticeType *lattice)
+{
+  const uint64_t n = (uint64_t)dims[0] * dims[1] * dims[2];
+  hipStream_t s;
+  HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  int *dlat                 = nullptr;
+  double *dsph              = nullptr;
+  int *dbox                 = nullptr;
+  unsigned long long *dcnt  = nullptr;
+  HIP_OK(hipMalloc(&dlat, n * sizeof(int)));
+  HIP_OK(hipMalloc(&dcnt, sizeof(unsigned long long)));
+  HIP_OK(hipMemsetAsync(dcnt, 0, sizeof(unsigned long long), s));
+  if (nSpheres > 0) {
+    HIP_OK(hipMalloc(&dsph, (size_t)nSpheres * 4 * sizeof(double)));
+    HIP_OK(hipMalloc(&dbox, (size_t)nSpheres * 6 * sizeof(int)));
+    HIP_OK(hipMemcpyAsync(dsph, spheres, (size_t)nSpheres * 4 * sizeof(double),
+        hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(dbox, bounds, (size_t)nSpheres * 6 * sizeof(int), hipMemcpyHostToDevice, s));
+  }
+  launchGeometry(dlat, dims, periodic, type == LBM_GEO_PIPE, type == LBM_GEO_BLOCKS ? blockSize : 0,
+      type == LBM_GEO_SPHERES ? nSpheres : 0, dsph, dbox, dcnt, s);
+  HIP_OK(hipGetLastError());
+  unsigned long long nfluid = 0;
+  HIP_OK(hipMemcpyAsync(&nfluid, dcnt, sizeof(nfluid), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipMemcpyAsync(lattice, dlat, n * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  HIP_OK(hipFree(dlat));
+  HIP_OK(hipFree(dcnt));
+  if (dsph)
+    HIP_OK(hipFree(dsph));
+  if (dbox)
+    HIP_OK(hipFree(dbox));
+  HIP_OK(hipStreamDestroy(s));
+  return (long long)nfluid;
 }
 
 void lbmMassMomentum(LatticeDescType *ld, KernelDataType *kd, double out[4])

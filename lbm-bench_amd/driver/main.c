@@ -58,12 +58,24 @@ int main(int argc, char *argv[])
 
   parseArguments(argc, argv, dims, &geometryType, &kernelToUse, periodic, &verify, &cd, &ro);
 
+  /* the kernel name is checked before anything runs, and the device is
+   * selected before geometryCreate, which builds the flags on it */
+  KernelFunctionsType *kf = NULL;
+  for (int j = 0; j < G_N_KERNELS; ++j)
+    if (!strcasecmp(kernelToUse, GKernels[j].Name)) {
+      kf = &GKernels[j];
+      break;
+    }
+  if (kf == NULL) {
+    printf("ERROR: kernel \"%s\" not found.\n", kernelToUse);
+    return 1;
+  }
+  if (ro.device >= 0 && lbmSelectDevice(ro.device) != 0)
+    exit(EXIT_FAILURE);
+
   geometrySetSeed(ro.seed);
   LatticeDescType ld;
   geometryCreate(geometryType, dims, periodic, &ld);
-
-  if (ro.device >= 0 && lbmSelectDevice(ro.device) != 0)
-    exit(EXIT_FAILURE);
   char dev[256] = "unknown";
   int cus       = 0;
   lbmDeviceInfo(dev, sizeof(dev), &cus, NULL);
@@ -85,16 +97,6 @@ int main(int argc, char *argv[])
   printf("#   rho out:           %e\n", (double)cd.RhoOut);
   printf("# - device:            %s, %d CUs\n", dev, cus);
 
-  KernelFunctionsType *kf = NULL;
-  for (int j = 0; j < G_N_KERNELS; ++j)
-    if (!strcasecmp(kernelToUse, GKernels[j].Name)) {
-      kf = &GKernels[j];
-      break;
-    }
-  if (kf == NULL) {
-    printf("ERROR: kernel \"%s\" not found.\n", kernelToUse);
-    return 1;
-  }
   printf("#\n# - kernel:            %s\n#\n", kf->Name);
 
   KernelDataType *kd = NULL;

@@ -121,6 +121,8 @@ class Library:
         L.kernelSetInitialVelocity.argtypes = [LDP, KDP, CDP]
         L.kernelAddBodyForce.argtypes = [KDP, LDP, CDP]
         L.kernelComputeBoundaryConditions.argtypes = [KDP, LDP, CDP]
+        L.lbmPlaneVelocity.argtypes = [LDP, KDP, c_int, dp]
+        L.lbmPlaneVelocity.restype = None
         L.lbmCellMoments.argtypes = [LDP, KDP, dp]
         L.lbmCellMoments.restype = None
         L.lbmSaveState.argtypes = [LDP, KDP, CDP, c_char_p]
@@ -239,6 +241,14 @@ class Lattice:
         out = np.empty(tuple(self.dims) + (4,), dtype=np.float64)
         self.lib.lib.lbmCellMoments(ctypes.byref(self.ld), self.kd,
                                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out
+
+    def plane_velocity(self, x):
+        """float64 (nY, nZ): x momentum of the logical PDFs of plane x
+        (lbmPlaneVelocity; 0 on obstacles)"""
+        out = np.empty((self.dims[1], self.dims[2]), dtype=np.float64)
+        self.lib.lib.lbmPlaneVelocity(ctypes.byref(self.ld), self.kd,
+                                      x, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         return out
 
     def save_state(self, path):

@@ -1,6 +1,8 @@
 /* host_sanitize_driver.c -- runs the host C of the library
- * (lbm-bench_amd/csrc/lbm_host.c: geometry, the -V initialisers,
- * statistics and verify; the Zou-He boundaries run on the device) under AddressSanitizer and
+ * (lbm-bench_amd/csrc/lbm_host.c: geometryCreate's parsing and sphere
+ * parameters, the -V initialisers, statistics and verify; the cell flags,
+ * the verification profile and the Zou-He boundaries are computed on the
+ * device, and host stand-ins below take the place of those entry points) under AddressSanitizer and
  * UndefinedBehaviorSanitizer (tests/test_host_sanitizers.py).
  *
  * lbm_host.c reaches the PDFs only through KernelDataType's accessors, so
@@ -46,6 +48,52 @@ void kernelDeinit(LatticeDescType *ld, KernelDataType **kd)
   (void)kd;
   fprintf(stderr, "kernelDeinit reached from host code\n");
   abort();
+}
+
+/* ---- stand-ins for the device entry points lbm_host.c calls -------------- */
+/* geometryCreate's flags: fluid inside, obstacles on the boundary layers of
+ * the non-periodic axes (the device builder adds pipe, blocks and spheres;
+ * here only their parameters are checked) */
+long long lbmBuildFlags(const int dims[3], const int periodic[3], int type, int blockSize,
+    int nSpheres, const double *spheres, const int *bounds, LatticeType *lattice)
+{
+  (void)type;
+  (void)blockSize;
+  long long fluid = 0;
+  for (int k = 0; k < nSpheres; ++k)
+    for (int a = 0; a < 3; ++a)
+      if (bounds[6 * k + a] > bounds[6 * k + 3 + a] || !(spheres[4 * k + 3] >= 4.0)) {
+        fprintf(stderr, "sphere %d: bad parameters\n", k);
+        abort();
+      }
+  for (int z = 0; z < dims[2]; ++z)
+    for (int y = 0; y < dims[1]; ++y)
+      for (int x = 0; x < dims[0]; ++x) {
+        const int face = (!periodic[0] && (x == 0 || x == dims[0] - 1)) ||
+                         (!periodic[1] && (y == 0 || y == dims[1] - 1)) ||
+                         (!periodic[2] && (z == 0 || z == dims[2] - 1));
+        lattice[latticeIndex((int *)dims, x, y, z)] = face ? LAT_CELL_OBSTACLE : LAT_CELL_FLUID;
+        fluid += !face;
+      }
+  return fluid;
+}
+
+static void getNode(KernelDataType *kd, int x, int y, int z, PdfType *p);
+
+/* the plane's x momentum through the mock accessors */
+void lbmPlaneVelocity(LatticeDescType *ld, KernelDataType *kd, int x, double *ux)
+{
+  PdfType p[Q];
+  for (int y = 0; y < ld->Dims[1]; ++y)
+    for (int z = 0; z < ld->Dims[2]; ++z) {
+      double v = 0.0;
+      if (ld->Lattice[latticeIndex(ld->Dims, x, y, z)] != LAT_CELL_OBSTACLE) {
+        getNode(kd, x, y, z, p);
+        v = p[D3Q19_E] + p[D3Q19_NE] + p[D3Q19_SE] + p[D3Q19_TE] + p[D3Q19_BE] - p[D3Q19_W] -
+            p[D3Q19_NW] - p[D3Q19_SW] - p[D3Q19_TW] - p[D3Q19_BW];
+      }
+      ux[(size_t)y * ld->Dims[2] + z] = v;
+    }
 }
 
 /* ---- the mock lattice: logical PDFs [cell][19], cell = latticeIndex ------ */

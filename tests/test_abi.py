@@ -135,28 +135,33 @@ def test_oracle_is_not_linked_by_the_product():
         assert "oracle" not in out
 
 
+# a LatticeDesc built from a numpy flag array (no geometryCreate, which needs
+# the device), then Init of the first registry entry in a child process
+_HAND_BUILT = (
+    "import sys, ctypes; sys.path.insert(0, %r); import numpy as np; import lbmhip; "
+    "lib = lbmhip.Library('dp'); L = lib.lib; ld = lib.LatticeDesc(); "
+    "flags = np.ones((6, 6, 6), dtype=np.int32); "
+    "ld.Dims = lbmhip.I3(6, 6, 6); ld.Lattice = flags.ctypes.data_as(ctypes.POINTER(ctypes.c_int)); "
+    "ld.nCells = ld.nFluid = 216; ld.PeriodicX = ld.PeriodicY = 1; ld.PeriodicZ = %d; "
+    "cd = lib.CaseData(); L.caseDataDefaults(ctypes.byref(cd)); cd.Omega = 1.0; %s; "
+    "kd = ctypes.POINTER(lib.KernelData)(); "
+    "lib.registry[0].Init(ctypes.byref(ld), ctypes.byref(kd), ctypes.byref(cd))")
+
+
 def test_storage_scale_must_be_power_of_two():
     """rejected at Init, before any device call"""
     import sys
-    code = ("import sys; sys.path.insert(0, %r); import lbmhip; "
-            "lib = lbmhip.Library('dp'); "
-            "lbmhip.Lattice(lib, 'aa-soa', 'channel', (8, 8, 8), (0, 0, 1), storage='fp16', scale=3.0)"
-            % os.path.dirname(lbmhip.__file__))
+    code = _HAND_BUILT % (os.path.dirname(lbmhip.__file__), 1,
+                          "cd.Storage = 2; cd.StorageScale = 3.0")
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert r.returncode != 0 and "power of two" in r.stderr
 
 
 def test_fluid_on_open_face_is_rejected():
-    """the kernels wrap every axis; a non-periodic face must be solid"""
+    """the kernels wrap every axis; a non-periodic face must be solid
+    (rejected at Init, before any device call)"""
     import sys
-    code = ("import sys, ctypes; sys.path.insert(0, %r); import lbmhip; "
-            "lib = lbmhip.Library('dp'); L = lib.lib; ld = lib.LatticeDesc(); "
-            "L.geometryCreate(b'box', lbmhip.I3(6, 6, 6), lbmhip.I3(1, 1, 1), ctypes.byref(ld)); "
-            "ld.PeriodicZ = 0; "
-            "cd = lib.CaseData(); L.caseDataDefaults(ctypes.byref(cd)); cd.Omega = 1.0; "
-            "kd = ctypes.POINTER(lib.KernelData)(); "
-            "lib.registry[0].Init(ctypes.byref(ld), ctypes.byref(kd), ctypes.byref(cd))"
-            % os.path.dirname(lbmhip.__file__))
+    code = _HAND_BUILT % (os.path.dirname(lbmhip.__file__), 0, "pass")
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert r.returncode != 0 and "non-periodic boundary" in r.stderr
 

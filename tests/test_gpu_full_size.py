@@ -133,3 +133,28 @@ def test_storage_forms_over_5000_steps(gpu_lib):
                   f"steps", flush=True)
             assert l2 <= u_bound, form
             assert drift <= m_bound, form
+
+
+def test_device_flags_equal_oracle_at_full_size(gpu_lib, oracle):
+    """geometryCreate builds the flags on the device (walls, 2000 spheres
+    wrapped through the periodic faces); they equal the oracle's host
+    restatement cell for cell, and the fluid count matches"""
+    with _lattice(gpu_lib) as L:
+        got = L.lattice_flags()
+        nf = L.n_fluid
+    exp, per, counts = oracle.geometry(GEOMETRY, DIMS, (0, 0, 1), 0)
+    assert per == (1, 0, 1)
+    assert counts[0] == nf == 128_407_050
+    assert np.array_equal(got, exp)
+
+
+def test_verify_reads_one_plane(gpu_lib):
+    """kernelVerify at 512^3 reduces the x = nX/2 plane on the device: the
+    norm is finite and no host mirror of the lattice (20.4 GB) is made, so
+    kd->PdfsActive stays NULL"""
+    with _lattice(gpu_lib) as L:
+        L.init("random", 0, 1.0, 1e-3)
+        L.run(2)
+        norm = L.verify()
+        assert not L.kd.contents.PdfsActive
+    assert np.isfinite(norm) and norm > 0

@@ -269,6 +269,26 @@ def test_channel_256_all_variants(gpu_lib, oracle, channel256):
         assert err < TOL, kernel
 
 
+@pytest.mark.parametrize("kernel", ["pull-soa", "aa-soa", "push-aos"])
+def test_plane_velocity_matches_nodes(gpu_lib, oracle, kernel):
+    """lbmPlaneVelocity (kernelVerify's profile, reduced on the device)
+    equals the x momentum of the logical PDFs summed in the reference's
+    order, bit for bit, after an even and an odd number of steps"""
+    dims = (12, 14, 70)
+    for steps in (3, 4):
+        with Lattice(gpu_lib, kernel, "spheres-6", dims, (0, 0, 1), omega=1.3, xforce=1e-4,
+                     body_force=True, magic=3 / 16, seed=3) as L:
+            L.init("random", 2, 1.0, 1e-2)
+            L.run(steps)
+            f = L.get_nodes()
+            for x in (0, 5, 11):
+                got = L.plane_velocity(x)
+                e = f[x]
+                exp = (e[..., 2] + e[..., 4] + e[..., 5] + e[..., 10] + e[..., 17] - e[..., 3]
+                       - e[..., 6] - e[..., 7] - e[..., 11] - e[..., 16])
+                assert np.array_equal(got, exp), (kernel, steps, x)
+
+
 def test_verification_case(gpu_lib):
     """LBM-Bench's -V case through the library: kernelSetInitial*, 1000
     steps with the force, kernelVerify; the norm the reference prints"""

@@ -1,7 +1,8 @@
 """The library's host C under AddressSanitizer and UndefinedBehaviorSanitizer.
 
-lbm-bench_amd/csrc/lbm_host.c (geometry for every type, the reference's -V
-initialisers, statistics and verify)
+lbm-bench_amd/csrc/lbm_host.c (geometryCreate's parsing and sphere
+parameters for every type, the reference's -V initialisers, statistics and
+verify; the device entry points it calls have host stand-ins)
 is compiled by gcc with tests/host_sanitize_driver.c, which drives it
 through a mock KernelDataType over a host array.  Ten lattices, including
 a one-cell-wide axis and a sphere packing with no fluid left.  A sanitizer
