@@ -44,6 +44,19 @@ void lbmMassMomentum(LatticeDescType *ld, KernelDataType *kd, double out[4]);
 void lbmCompareMomentum(LatticeDescType *ld, KernelDataType *a, KernelDataType *b,
     double out[4]);
 
+/* lbmCompareMomentum without allocating: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  The same four
+ * outputs, computed in one pass over both lattices (same dimensions, any
+ * storage forms, both on the current device) with the partial sums in a
+ * static device buffer of the library (1 MiB + 32 B per GPU, held until
+ * the process ends): no hipMalloc or hipFree.  Lattices of more than 2^28
+ * cells are refused.  The buffer is shared by every call on one GPU, so
+ * calls that overlap on one GPU race.  Returns 0, or -1 after a message
+ * on stderr (dimensions differ, lattice too large, no device). */
+int lbmCompareMomentumNoAlloc(LatticeDescType *ld, KernelDataType *a, KernelDataType *b,
+    double out[4]);
+
 /* Per-cell density and momentum: out[((x*nY + y)*nZ + z)*4 + k] holds rho,
  * jx, jy, jz of the logical PDFs (j is the velocity of the incompressible
  * model); obstacle cells hold 0.  The velocity field the reference can

@@ -598,7 +598,7 @@ __device__ __forceinline__ void blockReduce(double (&v)[K], int kmax)
 
 template <typename S>
 __global__ __launch_bounds__(256) void kMoments(const S *a, Geom g, int rule, float inv,
-    double *partials, double *field)
+    double rhoRef, double *partials, double *field)
 {
   using R  = typename Store<S>::R;
   using St = Store<S>;
@@ -617,10 +617,12 @@ __global__ __launch_bounds__(256) void kMoments(const S *a, Geom g, int rule, fl
     }
   }
   if (field != nullptr && c.cell < n) {
+    // the field holds the density itself: the deviation forms add back
+    // sum_d w_d RhoRef = RhoRef (j needs nothing: sum_d c_d w_d = 0)
     const double nan = __builtin_nan("");
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-      field[k * n + c.cell] = c.valid ? m[k] : nan;
+      field[k * n + c.cell] = c.valid ? m[k] + (k == 0 && St::scaled ? rhoRef : 0.0) : nan;
   }
   blockReduce<4>(m, -1);
   if (threadIdx.x == 0)
@@ -640,7 +642,7 @@ int launchMoments(const AuxLaunch &A, double *partials, double *field, hipStream
   const int blocks = momentsBlocks(A.g);
 #define LAUNCH(T)                                                                          \
   hipLaunchKernelGGL(kMoments<T>, dim3(blocks), dim3(256), 0, s, (const T *)A.pdf, A.g, A.rule,   \
-      A.invScale, partials, field)
+      A.invScale, A.rhoRef, partials, field)
   switch (A.storage) {
   case ST_F64: LAUNCH(double); break;
   case ST_F32: LAUNCH(float); break;
@@ -711,6 +713,84 @@ int launchCompareFields(const double *fa, const double *fb, uint64_t n, double *
 void launchReduceCompare(const double *partials, int n, double *out, hipStream_t s)
 {
   hipLaunchKernelGGL(kReduceK<4>, dim3(1), dim3(256), 0, s, partials, n, out, 2);
+}
+
+// ---- comparison of two lattices without field buffers -------------------------
+// The moments of kMoments, per cell of two lattices of the same dimensions
+// (any storage forms), and the four terms of kCompare, in one pass: each
+// workgroup sums its chunk of `chunk` consecutive cells into one slot of
+// partials.  A cell counts when it is fluid in both lattices.
+__device__ __forceinline__ double rawLoad(const void *pdf, int storage, uint64_t i, float inv)
+{
+  switch (storage) {
+  case ST_F64: return ((const double *)pdf)[i];
+  case ST_F32: return (double)((const float *)pdf)[i];
+  case ST_F32DEV: return (double)(((const float *)pdf)[i] * inv);
+  case ST_F16: return (double)(__half2float(((const __half *)pdf)[i]) * inv);
+  default: return (double)(__bfloat162float(((const __hip_bfloat16 *)pdf)[i]) * inv);
+  }
+}
+
+struct CompareSide {
+  const void *pdf;
+  Geom g;
+  int storage, rule;
+  double rhoRef; // added to rho for the deviation forms
+  float inv;
+};
+
+__device__ __forceinline__ void cellMoments(const CompareSide &L, int x, int y, int z,
+    uint64_t cell, double (&m)[4])
+{
+  m[0] = m[1] = m[2] = m[3] = 0.0;
+#pragma unroll
+  for (int d = 0; d < Q; ++d) {
+    const double v = rawLoad(L.pdf, L.storage, logicalSlot(L.g, L.rule, x, y, z, cell, d), L.inv);
+    m[0] += v;
+    m[1] += cx(d) * v;
+    m[2] += cy(d) * v;
+    m[3] += cz(d) * v;
+  }
+  m[0] += L.storage >= ST_F32DEV ? L.rhoRef : 0.0;
+}
+
+__global__ __launch_bounds__(256) void kCompareLattices(CompareSide A, CompareSide B, uint64_t n,
+    uint64_t chunk, double *partials)
+{
+  double v[4] = { 0.0, 0.0, 0.0, 0.0 };
+  const uint64_t first = (uint64_t)blockIdx.x * chunk;
+  const uint64_t end   = first + chunk < n ? first + chunk : n;
+  for (uint64_t c = first + threadIdx.x; c < end; c += 256u) {
+    const int z      = (int)(c % A.g.nz);
+    const uint64_t r = c / A.g.nz;
+    const int y = (int)(r % A.g.ny), x = (int)(r / A.g.ny);
+    if (solidAt(A.g.mask, A.g.segs, A.g.ny, x, y, z) || solidAt(B.g.mask, B.g.segs, B.g.ny, x, y, z))
+      continue;
+    double a[4], b[4];
+    cellMoments(A, x, y, z, c, a);
+    cellMoments(B, x, y, z, c, b);
+    const double dx = a[1] - b[1], dy = a[2] - b[2], dz = a[3] - b[3];
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    v[0] += d2;
+    v[1] += b[1] * b[1] + b[2] * b[2] + b[3] * b[3];
+    v[2] = fmax(v[2], sqrt(d2));
+    v[3] += (a[0] - b[0]) * (a[0] - b[0]);
+  }
+  blockReduce<4>(v, 2);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      partials[4 * (uint64_t)blockIdx.x + k] = v[k];
+}
+
+void launchCompareLattices(const AuxLaunch &A, const AuxLaunch &B, uint64_t chunk,
+    double *partials, hipStream_t s)
+{
+  const uint64_t n = (uint64_t)A.g.nx * A.g.ny * A.g.nz;
+  const CompareSide a{ A.pdf, A.g, A.storage, A.rule, A.rhoRef, A.invScale };
+  const CompareSide b{ B.pdf, B.g, B.storage, B.rule, B.rhoRef, B.invScale };
+  hipLaunchKernelGGL(kCompareLattices, dim3((unsigned)((n + chunk - 1) / chunk)), dim3(256), 0, s,
+      a, b, n, chunk, partials);
 }
 
 // ---- copy ------------------------------------------------------------------

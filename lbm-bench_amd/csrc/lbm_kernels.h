@@ -106,6 +106,11 @@ void launchReduce4(const double *partials, int n, double *out, hipStream_t s);
 int launchCompareFields(const double *fa, const double *fb, uint64_t ncells, double *partials,
     hipStream_t s);
 void launchReduceCompare(const double *partials, int n, double *out, hipStream_t s);
+// the comparison above straight from two lattices of the same dimensions
+// (no field buffers): partials[4 * k ...] of the k-th chunk of `chunk`
+// cells, ceil(cells / chunk) chunks
+void launchCompareLattices(const AuxLaunch &A, const AuxLaunch &B, uint64_t chunk,
+    double *partials, hipStream_t s);
 // plain device copy, for the measured bandwidth line and counter scaling
 void launchCopy(const double *src, double *dst, uint64_t n, hipStream_t s);
 // porous solid-fraction field: uniform noise from the splitmix64 chain over

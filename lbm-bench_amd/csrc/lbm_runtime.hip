@@ -105,6 +105,43 @@ struct LbmDevice {
   ZouHeLaunch zh;
 };
 
+// ---- device scratch of the no-allocation entry point ----------------------------
+// lbmCompareMomentumNoAlloc keeps its partial sums here instead of in
+// allocated memory: one slot of 4 doubles {sum |dj|^2, sum |jB|^2,
+// max |dj|, sum drho^2} per chunk of kNoAllocChunk cells, then the 4
+// totals.  4 * 32768 + 4 doubles = 1 MiB + 32 B, so lattices of up to
+// 32768 * 8192 = 2^28 cells.  Each GPU the library runs on holds its own
+// copy until the process ends.  In host code the name `scratch` is a
+// host-side stand-in, not a device address: the address comes from
+// hipGetSymbolAddress, once per device with that device current
+// (ensureScratch, called from Init), and only that address is used.
+constexpr uint64_t kNoAllocChunk  = 8192;
+constexpr uint64_t kNoAllocGroups = 32768;
+constexpr int kMaxDevices         = 64;
+static __device__ __align__(256) double scratch[4 * kNoAllocGroups + 4];
+static double *gScratch[kMaxDevices]; // device address of scratch, per device
+
+// the current device's scratch address (looked up on first use on a device);
+// null after a message when there is none
+static double *ensureScratch()
+{
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    fprintf(stderr, "ERROR: no usable HIP device for the scratch buffer\n");
+    return nullptr;
+  }
+  if (!gScratch[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      fprintf(stderr, "ERROR: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+      return nullptr;
+    }
+    gScratch[dev] = (double *)p;
+  }
+  return gScratch[dev];
+}
+
 // ---- small helpers ---------------------------------------------------------
 static int envInt(const char *name, int dflt)
 {
@@ -809,6 +846,8 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
   HIP_OK(hipStreamSynchronize(D->stream));
   HIP_OK(hipFree(dlat));
 
+  if (!ensureScratch())
+    fail("the device scratch buffer has no address on this device");
   D->npart = momentsBlocks(geomOf(D));
   HIP_OK(hipMalloc(&D->dpart, sizeof(double) * 4 * (size_t)D->npart));
   HIP_OK(hipMalloc(&D->dout, sizeof(double) * 4));
@@ -1018,6 +1057,39 @@ void lbmCompareMomentum(LatticeDescType *ld, KernelDataType *a, KernelDataType *
   HIP_OK(hipStreamSynchronize(A->stream));
   HIP_OK(hipFree(fa));
   HIP_OK(hipFree(fb));
+}
+
+int lbmCompareMomentumNoAlloc(LatticeDescType *ld, KernelDataType *a, KernelDataType *b,
+    double out[4])
+{
+  LbmDevice *A = a->Dev, *B = b->Dev;
+  if (A->nx != B->nx || A->ny != B->ny || A->nz != B->nz) {
+    fprintf(stderr, "ERROR: lbmCompareMomentumNoAlloc: lattices of %d x %d x %d and %d x %d x %d "
+                    "cells\n", A->nx, A->ny, A->nz, B->nx, B->ny, B->nz);
+    return -1;
+  }
+  if (A->ncells > kNoAllocChunk * kNoAllocGroups) {
+    fprintf(stderr, "ERROR: lbmCompareMomentumNoAlloc: %llu cells exceed the %llu its scratch "
+                    "buffer covers\n", (unsigned long long)A->ncells,
+        (unsigned long long)(kNoAllocChunk * kNoAllocGroups));
+    return -1;
+  }
+  double *sp = ensureScratch();
+  if (!sp)
+    return -1;
+  flushMirror(A);
+  flushMirror(B);
+  HIP_OK(hipStreamSynchronize(B->stream));
+  const uint64_t groups = (A->ncells + kNoAllocChunk - 1) / kNoAllocChunk;
+  launchCompareLattices(auxOf(A, currentRule(A, a)), auxOf(B, currentRule(B, b)), kNoAllocChunk,
+      sp, A->stream);
+  HIP_OK(hipGetLastError());
+  launchReduceCompare(sp, (int)groups, sp + 4 * kNoAllocGroups, A->stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(out, sp + 4 * kNoAllocGroups, 4 * sizeof(double), hipMemcpyDeviceToHost,
+      A->stream));
+  HIP_OK(hipStreamSynchronize(A->stream));
+  return 0;
 }
 
 void lbmGetNodes(LatticeDescType *ld, KernelDataType *kd, double *out)

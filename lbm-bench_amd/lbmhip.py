@@ -101,6 +101,8 @@ class Library:
         L.lbmInitPdfs.argtypes = [LDP, KDP, c_int, ctypes.c_uint64, c_double, c_double]
         L.lbmMassMomentum.argtypes = [LDP, KDP, dp]
         L.lbmCompareMomentum.argtypes = [LDP, KDP, KDP, dp]
+        L.lbmCompareMomentumNoAlloc.argtypes = [LDP, KDP, KDP, dp]
+        L.lbmCompareMomentumNoAlloc.restype = c_int
         L.lbmRunSteps.argtypes = [LDP, KDP, CDP, c_int]
         L.lbmRunSteps.restype = c_double
         L.lbmDeviceBytes.argtypes = [KDP]
@@ -342,6 +344,16 @@ class Lattice:
         out = np.zeros(4)
         self.lib.lib.lbmCompareMomentum(ctypes.byref(self.ld), self.kd, other.kd,
                                         out.ctypes.data_as(ctypes.POINTER(c_double)))
+        return out
+
+    def compare_momentum_noalloc(self, other):
+        """lbmCompareMomentumNoAlloc: compare_momentum without device
+        allocations; raises ValueError when the library refuses the call"""
+        out = np.zeros(4)
+        rc = self.lib.lib.lbmCompareMomentumNoAlloc(ctypes.byref(self.ld), self.kd, other.kd,
+                                                    out.ctypes.data_as(ctypes.POINTER(c_double)))
+        if rc != 0:
+            raise ValueError("lbmCompareMomentumNoAlloc refused the lattices")
         return out
 
     def device_bytes(self):
