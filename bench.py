@@ -37,13 +37,33 @@ def log(msg):
 
 
 def traffic_from_profiles(kernel_tag):
-    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/*.md)"""
+    """HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE runs summarised in profiles/traffic.json (not
+    measured by this run: a profiler cannot run inside it) -> (bytes, source)"""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
-        return None
+        return None, None
     with open(path) as f:
         data = json.load(f)
-    return data.get(kernel_tag)
+    entry = data.get(kernel_tag)
+    if entry is None:
+        return None, None
+    return entry["bytes"], entry["source"]
+
+
+# the copy shapes of the achievable-bandwidth sweep: 16-byte vectors per lane,
+# non-temporal or cached, one workgroup per tile or a persistent grid
+COPY_SHAPES = [(v, nt, pg) for v in (1, 2, 4, 8) for nt in (True, False) for pg in (0, 2048)]
+
+
+def copy_sweep(lib, nbytes=4 << 30, reps=8):
+    """GB/s of every copy shape; the best one is the achievable line"""
+    rows = []
+    for v, nt, pg in COPY_SHAPES:
+        gbs = lib.copy_bandwidth_shape(nbytes, reps, v, nt, pg)
+        rows.append({"vectors_per_lane": v, "non_temporal": nt, "persistent_groups": pg,
+                     "gbs": round(gbs, 1)})
+    return rows
 
 
 def cpu_baseline():
@@ -85,6 +105,7 @@ def adjoint_run(lib, lbmhip):
         J, g, stats = A.gradient(T, [32])
         total = stats["forward_s"] + stats["recompute_s"] + stats["adjoint_s"]
         key = "fp64" if form == "native" else "fp32"
+        # whole gradient: forward sweep, recomputation and adjoint sweep
         out[key] = {
             "J": J,
             "forward_step_ms": round(st["forward_s"] * 1e3, 4),
@@ -104,6 +125,13 @@ def adjoint_run(lib, lbmhip):
             f"{out[key]['mlups_gradient']} MLUPS, {out[key]['device_gib']} GiB")
         A.close()
     return out
+
+
+def dominant_name(kernel, kind):
+    """the HIP kernel behind a step kind of the bench's default settings"""
+    if kernel == "aa-soa":
+        return "kRowSweep<AA odd, FP64>" if kind == "odd" else "kStepRow<AA even, FP64>"
+    return f"{kernel} step (FP64)"
 
 
 def cpu_model():
@@ -339,6 +367,9 @@ def main():
 
     copy_gbs = lib.copy_bandwidth(4 << 30, 10)
     log(f"device copy bandwidth {copy_gbs:.0f} GB/s")
+    sweep = copy_sweep(lib)
+    best_copy = max(sweep, key=lambda r: r["gbs"])
+    log(f"best copy shape {best_copy}")
     # the AA step's loads and stores without arithmetic, on this lattice's size
     pattern_gbs = lib.pattern_bandwidth(19, 8, True, n ** 3, 6)
     log(f"19-plane in-place pattern bandwidth {pattern_gbs:.0f} GB/s")
@@ -359,6 +390,20 @@ def main():
     wall = time.perf_counter() - t0
     m1 = L64.mass_momentum()
     mlups = nfluid * args.steps / dur / 1e6
+    # per-kernel durations: the same steps launched one by one with a hipEvent
+    # after each launch on the library's stream (aa-soa: even and odd steps)
+    _, kt = L64.run_timed(args.steps)
+    kernels = {}
+    for kind, (secs, count) in kt.items():
+        if count:
+            ms_k = secs / count * 1e3
+            gbs_k = BYTES_PER_LUP["native"] * nfluid / (secs / count) / 1e9
+            kernels[kind] = {"launches": count, "avg_ms": round(ms_k, 4),
+                             "achieved_gbs": round(gbs_k, 1),
+                             "frac": round(gbs_k / HBM_PEAK_GBS, 4)}
+    dominant = max(kernels, key=lambda k: kernels[k]["avg_ms"])
+    dom = kernels[dominant]
+    log(f"kernels {kernels}; dominant {dominant}")
     step_s = dur / args.steps
     achieved = BYTES_PER_LUP["native"] * nfluid / step_s / 1e9
     log(f"fp64 {args.kernel}: {mlups:.0f} MLUPS, {step_s * 1e3:.3f} ms/step, "
@@ -391,6 +436,7 @@ def main():
     L64.close()
 
     adjoint = None if args.no_adjoint else adjoint_run(lib, lbmhip)
+    traffic = traffic_from_profiles(f"{args.kernel}-fp64-{n}-{dominant}")
 
     cpu = None if args.no_cpu else cpu_baseline()
     cpu_omp = None if args.no_cpu else cpu_reference_openmp()
@@ -413,15 +459,28 @@ def main():
                                f"body force 1e-6", "kernel": args.kernel, "lattice": list(dims),
                    "fluid_cells": nfluid, "solid_fraction": round(solid_fraction, 5),
                    "storage": "fp64"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles(f"{args.kernel}-fp64-{n}")},
+        "roofline": {"bound": "hbm", "achieved": dom["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": dom["frac"],
+                     "kernel": dominant_name(args.kernel, dominant),
+                     "kernel_avg_ms": dom["avg_ms"],
+                     "bytes_per_launch": BYTES_PER_LUP["native"] * nfluid,
+                     "traffic": traffic[0], "traffic_source": traffic[1],
+                     "step_pair_achieved": round(achieved, 1),
+                     "step_pair_frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "achievable_gbs": best_copy["gbs"],
+                     "achievable_shape": {k: best_copy[k] for k in ("vectors_per_lane",
+                                                                    "non_temporal",
+                                                                    "persistent_groups")},
+                     "frac_of_achievable": round(dom["achieved_gbs"] / best_copy["gbs"], 4)},
+        "kernels": kernels,
+        "copy_sweep_gbs": sweep,
         "cpu_baseline": cpu,
         "cpu_reference_openmp": cpu_omp,
         "fp16": forms.get("fp16"),
         "fp32": forms.get("fp32"),
         "bf16": forms.get("bf16"),
-        "adjoint_mlups": adjoint["fp64"]["mlups_adjoint_sweep"] if adjoint else None,
+        "adjoint_mlups": adjoint["fp64"]["mlups_gradient"] if adjoint else None,
+        "adjoint_sweep_mlups": adjoint["fp64"]["mlups_adjoint_sweep"] if adjoint else None,
         "adjoint": adjoint,
         "mass_drift_rel": (m1[0] - m0[0]) / m0[0],
         "copy_bandwidth_gbs": round(copy_gbs, 1),

@@ -107,6 +107,16 @@ int lbmLoadState(LatticeDescType *ld, KernelDataType *kd, const char *path);
  * hipEvent time of the launches in seconds. */
 double lbmRunSteps(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd, int steps);
 
+/* lbmRunSteps with every step launched on its own (no graph replay) and a
+ * hipEvent recorded on the library's stream after each launch: returns the
+ * run's time as lbmRunSteps does, and kernelSeconds[0] / launches[0] the
+ * summed durations and number of the step launches (aa-soa: its even
+ * steps), kernelSeconds[1] / launches[1] those of aa-soa's odd steps (0
+ * for the other variants).  A pressure-boundary pass after a step counts
+ * with that step.  For a per-kernel roofline. */
+double lbmRunStepsTimed(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd, int steps,
+    double kernelSeconds[2], int launches[2]);
+
 /* bytes of device memory held by the PDF arrays of kd (with push-aos's
  * post-collision planes when it runs in two passes) */
 size_t lbmDeviceBytes(KernelDataType *kd);
@@ -115,6 +125,15 @@ size_t lbmDeviceBytes(KernelDataType *kd);
  * stream; returns GB/s (read + write bytes / second) from hipEvents.  The
  * achievable-bandwidth line beside the roofline. */
 double lbmCopyBandwidth(size_t bytes, int reps);
+
+/* lbmCopyBandwidth with the copy's shape chosen: `vectors` 16-byte vectors
+ * per lane (1, 2, 4 or 8; lbmCopyBandwidth uses 4), nonTemporal != 0 for
+ * non-temporal loads and stores (else cached), persistentGroups > 0 for a
+ * persistent grid of that many workgroups striding over the array (0: one
+ * workgroup per tile).  Returns GB/s, or -1 after a message on stderr for
+ * another shape.  The sweep behind the achievable line. */
+double lbmCopyBandwidthShape(size_t bytes, int reps, int vectors, int nonTemporal,
+    int persistentGroups);
 
 /* The memory traffic of one SoA step without its arithmetic, as GB/s (read
  * + write bytes / second, hipEvents over `reps` launches): every lane loads

@@ -824,6 +824,65 @@ void launchCopy(const double *src, double *dst, uint64_t n, hipStream_t s)
   hipLaunchKernelGGL(kCopy, dim3(blocks), dim3(256), 0, s, (const v2d *)src, (v2d *)dst, n2);
 }
 
+// ---- copy kernels of other shapes (the achievable-bandwidth sweep) -------------
+// V 16-byte vectors per lane, 256 lanes per workgroup, workgroup b copying
+// V * 256 consecutive vectors; NT: non-temporal loads and stores;
+// persistent: a grid of `groups` workgroups strides over the array.  All
+// loads of a lane are issued before its stores.
+template <int V, bool NT>
+__global__ __launch_bounds__(256) void kCopyShape(const v2d *__restrict__ src, v2d *__restrict__ dst,
+    uint64_t n2, int persistent)
+{
+  const uint64_t tile   = (uint64_t)V * 256u;
+  const uint64_t stride = persistent ? (uint64_t)gridDim.x * tile : 0;
+  for (uint64_t base = (uint64_t)blockIdx.x * tile + threadIdx.x; base < n2;
+       base += persistent ? stride : n2) {
+    v2d v[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      const uint64_t i = base + 256u * k;
+      if (i < n2)
+        v[k] = NT ? __builtin_nontemporal_load(&src[i]) : src[i];
+    }
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      const uint64_t i = base + 256u * k;
+      if (i < n2) {
+        if (NT)
+          __builtin_nontemporal_store(v[k], &dst[i]);
+        else
+          dst[i] = v[k];
+      }
+    }
+  }
+}
+
+bool launchCopyShape(const double *src, double *dst, uint64_t n, int vectors, int nt,
+    int persistentGroups, hipStream_t s)
+{
+  const uint64_t n2     = n / 2;
+  const uint64_t tile   = (uint64_t)vectors * 256u;
+  const unsigned blocks = persistentGroups > 0 ? (unsigned)persistentGroups
+                                               : (unsigned)((n2 + tile - 1) / tile);
+  const int persistent  = persistentGroups > 0;
+  const v2d *a = (const v2d *)src;
+  v2d *b       = (v2d *)dst;
+#define LAUNCH(V, NTV)                                                                     \
+  hipLaunchKernelGGL((kCopyShape<V, NTV>), dim3(blocks), dim3(256), 0, s, a, b, n2, persistent)
+  switch (vectors * 2 + (nt ? 1 : 0)) {
+  case 2: LAUNCH(1, false); return true;
+  case 3: LAUNCH(1, true); return true;
+  case 4: LAUNCH(2, false); return true;
+  case 5: LAUNCH(2, true); return true;
+  case 8: LAUNCH(4, false); return true;
+  case 9: LAUNCH(4, true); return true;
+  case 16: LAUNCH(8, false); return true;
+  case 17: LAUNCH(8, true); return true;
+  default: return false;
+  }
+#undef LAUNCH
+}
+
 // ---- porous solid-fraction field (differentiation form) ---------------------
 // The smoothing passes compute each output as the host formula does, tap by
 // tap from -rad to rad without fused multiply-adds, so the smoothed field

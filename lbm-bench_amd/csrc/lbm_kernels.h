@@ -113,6 +113,11 @@ void launchCompareLattices(const AuxLaunch &A, const AuxLaunch &B, uint64_t chun
     double *partials, hipStream_t s);
 // plain device copy, for the measured bandwidth line and counter scaling
 void launchCopy(const double *src, double *dst, uint64_t n, hipStream_t s);
+// copies of other shapes: `vectors` 16-byte vectors per lane (1, 2, 4, 8),
+// non-temporal or cached, one workgroup per tile or a persistent grid of
+// persistentGroups workgroups (0: one per tile); false for other shapes
+bool launchCopyShape(const double *src, double *dst, uint64_t n, int vectors, int nt,
+    int persistentGroups, hipStream_t s);
 // porous solid-fraction field: uniform noise from the splitmix64 chain over
 // (seed, cell), one periodic Gaussian pass per axis (field [x][y][z], taps
 // kern[0 .. 2 rad]), and s = 1 / (1 + exp(-(a (s - mean) / sd + b)))

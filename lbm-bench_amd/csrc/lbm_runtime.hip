@@ -103,6 +103,11 @@ struct LbmDevice {
   uint8_t *dface;
   int zhOn;
   ZouHeLaunch zh;
+  // per-launch timing (lbmRunStepsTimed): an event after every step launch,
+  // the first one before it; tkind: 1 for an AA odd step, else 0
+  bool timing;
+  std::vector<hipEvent_t> tev;
+  std::vector<int> tkind;
 };
 
 // ---- device scratch of the no-allocation entry point ----------------------------
@@ -466,6 +471,13 @@ static void launchStep(LbmDevice *D, StepLaunch L, int variant, int srcIdx, int 
     launchZouHe(A, Z, s);
     HIP_OK(hipGetLastError());
   }
+  if (D->timing) {
+    hipEvent_t e;
+    HIP_OK(hipEventCreate(&e));
+    HIP_OK(hipEventRecord(e, s));
+    D->tev.push_back(e);
+    D->tkind.push_back(variant == V_AA_ODD ? 1 : 0);
+  }
 }
 
 static int stepVariant(const LbmDevice *D) { return D->prop == PROP_PULL ? V_PULL : V_PUSH; }
@@ -573,7 +585,12 @@ static ZouHeLaunch zouHeOf(const LbmDevice *D, const CaseDataType *cd, const Ste
   return Z;
 }
 
-static void runKernel(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cdIn)
+// timed != null: every step is launched eagerly (no graph), with a hipEvent
+// after each launch; timed[0] / timed[1] receive the summed durations of
+// the step launches (AA: even steps) and of the AA odd steps, counts[0..1]
+// their numbers
+static void runKernel(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cdIn,
+    double *timed = nullptr, int *counts = nullptr)
 {
   const CaseDataType eff = caseDataEffective(cdIn);
   const CaseDataType *cd = &eff;
@@ -592,12 +609,20 @@ static void runKernel(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cdI
   if (D->zhOn)
     D->zh = zouHeOf(D, cd, base, bf);
   prepareGraphs(D, cd, base);
-  const int G = D->gvalid[0] ? D->gsteps : 0;
+  const int G = D->gvalid[0] && !timed ? D->gsteps : 0;
 
   // the reference's LIKWID region around the kernel (main.c:122-131), as a
   // ROCm marker range: rocprofv3 --marker-trace shows it
   roctxRangePush("PROPKERNEL");
   HIP_OK(hipEventRecord(D->ev0, D->stream));
+  if (timed) {
+    D->timing = true;
+    hipEvent_t e;
+    HIP_OK(hipEventCreate(&e));
+    HIP_OK(hipEventRecord(e, D->stream));
+    D->tev.assign(1, e);
+    D->tkind.assign(1, -1);
+  }
   if (n > 0) {
     int left = n;
     if (D->prop == PROP_AA) {
@@ -639,6 +664,21 @@ static void runKernel(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cdI
   HIP_OK(hipEventSynchronize(D->ev1));
   roctxRangePop();
   D->zhOn = 0;
+  if (timed) {
+    timed[0] = timed[1] = 0.0;
+    counts[0] = counts[1] = 0;
+    for (size_t i = 1; i < D->tev.size(); ++i) {
+      float t = 0.0f;
+      HIP_OK(hipEventElapsedTime(&t, D->tev[i - 1], D->tev[i]));
+      timed[D->tkind[i]] += (double)t * 1e-3;
+      counts[D->tkind[i]] += 1;
+    }
+    for (hipEvent_t e : D->tev)
+      HIP_OK(hipEventDestroy(e));
+    D->tev.clear();
+    D->tkind.clear();
+    D->timing = false;
+  }
   float ms = 0.0f;
   HIP_OK(hipEventElapsedTime(&ms, D->ev0, D->ev1));
   kd->Duration = (double)ms * 1e-3;
@@ -781,6 +821,7 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
   D->dseqHost      = 0;
   D->dface         = nullptr;
   D->zhOn          = 0;
+  D->timing        = false;
   kd->LoopBalance = 2.0 * Q * (double)D->esize;
 
   HIP_OK(hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking));
@@ -1347,6 +1388,16 @@ void lbmInitPdfs(LatticeDescType *ld, KernelDataType *kd, int mode, uint64_t see
   D->mirrorDirty = false;
 }
 
+double lbmRunStepsTimed(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd, int steps,
+    double kernelSeconds[2], int launches[2])
+{
+  const int keep    = cd->MaxIterations;
+  cd->MaxIterations = steps;
+  runKernel(ld, kd, cd, kernelSeconds, launches);
+  cd->MaxIterations = keep;
+  return kd->Duration;
+}
+
 double lbmRunSteps(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd, int steps)
 {
   const int keep    = cd->MaxIterations;
@@ -1449,6 +1500,44 @@ void lbmPorousField(const int dims[3], uint64_t seed, double sigma, double a, do
   HIP_OK(hipFree(u));
   HIP_OK(hipFree(t));
   HIP_OK(hipStreamDestroy(st));
+}
+
+double lbmCopyBandwidthShape(size_t bytes, int reps, int vectors, int nonTemporal,
+    int persistentGroups)
+{
+  const bool ok = (vectors == 1 || vectors == 2 || vectors == 4 || vectors == 8) && reps >= 1 &&
+                  persistentGroups >= 0 && bytes >= 16;
+  if (!ok) {
+    fprintf(stderr, "lbmCopyBandwidthShape: vectors must be 1, 2, 4 or 8, reps positive, "
+                    "persistentGroups >= 0, bytes >= 16 (got %d, %d, %d, %zu)\n",
+        vectors, reps, persistentGroups, bytes);
+    return -1.0;
+  }
+  double *a = nullptr, *b = nullptr;
+  const uint64_t n = bytes / sizeof(double);
+  hipStream_t s;
+  hipEvent_t e0, e1;
+  HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  HIP_OK(hipMalloc(&a, n * sizeof(double)));
+  HIP_OK(hipMalloc(&b, n * sizeof(double)));
+  HIP_OK(hipMemsetAsync(a, 0, n * sizeof(double), s));
+  launchCopyShape(a, b, n, vectors, nonTemporal, persistentGroups, s);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r)
+    launchCopyShape(a, b, n, vectors, nonTemporal, persistentGroups, s);
+  HIP_OK(hipEventRecord(e1, s));
+  HIP_OK(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+  HIP_OK(hipFree(a));
+  HIP_OK(hipFree(b));
+  HIP_OK(hipEventDestroy(e0));
+  HIP_OK(hipEventDestroy(e1));
+  HIP_OK(hipStreamDestroy(s));
+  return 2.0 * (double)(n / 2 * 2) * sizeof(double) * reps / ((double)ms * 1e-3) / 1e9;
 }
 
 double lbmPatternBandwidth(int planes, int elementBytes, int inPlace, size_t cells, int reps)

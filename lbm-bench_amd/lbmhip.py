@@ -103,12 +103,16 @@ class Library:
         L.lbmCompareMomentum.argtypes = [LDP, KDP, KDP, dp]
         L.lbmCompareMomentumNoAlloc.argtypes = [LDP, KDP, KDP, dp]
         L.lbmCompareMomentumNoAlloc.restype = c_int
+        L.lbmRunStepsTimed.argtypes = [LDP, KDP, CDP, c_int, dp, ctypes.POINTER(c_int)]
+        L.lbmRunStepsTimed.restype = c_double
         L.lbmRunSteps.argtypes = [LDP, KDP, CDP, c_int]
         L.lbmRunSteps.restype = c_double
         L.lbmDeviceBytes.argtypes = [KDP]
         L.lbmDeviceBytes.restype = ctypes.c_size_t
         L.lbmCopyBandwidth.argtypes = [ctypes.c_size_t, c_int]
         L.lbmCopyBandwidth.restype = c_double
+        L.lbmCopyBandwidthShape.argtypes = [ctypes.c_size_t, c_int, c_int, c_int, c_int]
+        L.lbmCopyBandwidthShape.restype = c_double
         L.lbmPatternBandwidth.argtypes = [c_int, c_int, c_int, ctypes.c_size_t, c_int]
         L.lbmPatternBandwidth.restype = c_double
         L.lbmDeviceSynchronize.restype = c_int
@@ -164,6 +168,14 @@ class Library:
 
     def copy_bandwidth(self, nbytes, reps):
         return self.lib.lbmCopyBandwidth(nbytes, reps)
+
+    def copy_bandwidth_shape(self, nbytes, reps, vectors, non_temporal, persistent_groups=0):
+        """GB/s of a copy of the given shape (lbmCopyBandwidthShape)"""
+        gbs = self.lib.lbmCopyBandwidthShape(nbytes, reps, vectors, int(bool(non_temporal)),
+                                             persistent_groups)
+        if gbs < 0:
+            raise ValueError("copy_bandwidth_shape: unsupported shape")
+        return gbs
 
     def pattern_bandwidth(self, planes, element_bytes, in_place, cells, reps):
         """GB/s of the step's loads and stores without arithmetic"""
@@ -320,6 +332,17 @@ class Lattice:
         self.cd.MaxIterations = steps
         self.kd.contents.Kernel(ctypes.byref(self.ld), self.kd, ctypes.byref(self.cd))
         return self.kd.contents.Duration
+
+    def run_timed(self, steps):
+        """runs `steps` steps, each launched on its own with a hipEvent after it
+        -> (run seconds, {"step": (seconds, launches), "odd": (seconds, launches)})
+        where "step" are the step launches (aa-soa: even steps) and "odd"
+        aa-soa's odd steps"""
+        t = (c_double * 2)()
+        n = (c_int * 2)()
+        total = self.lib.lib.lbmRunStepsTimed(ctypes.byref(self.ld), self.kd, ctypes.byref(self.cd),
+                                              steps, t, n)
+        return total, {"step": (t[0], n[0]), "odd": (t[1], n[1])}
 
     def add_body_force(self):
         self.lib.lib.kernelAddBodyForce(self.kd, ctypes.byref(self.ld), ctypes.byref(self.cd))

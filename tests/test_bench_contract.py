@@ -38,7 +38,17 @@ def test_bench_line_has_the_contract_fields():
     roof = out["roofline"]
     assert roof["bound"] == "hbm" and roof["unit"] == "GB/s" and roof["peak"] == 8000.0
     assert abs(roof["frac"] - roof["achieved"] / roof["peak"]) < 1e-3
-    # 304 algorithmic bytes per fluid-cell update, at the measured rate
-    assert abs(roof["achieved"] - out["value"] * 304 / 1e3) < 0.01 * roof["achieved"]
+    # the dominant kernel: 304 algorithmic bytes per fluid-cell update over
+    # its own mean launch time (hipEvents on the library's stream)
+    nf = out["config"]["fluid_cells"]
+    assert roof["bytes_per_launch"] == 304 * nf
+    assert abs(roof["achieved"] - roof["bytes_per_launch"] / roof["kernel_avg_ms"] / 1e6) \
+        < 1e-3 * roof["achieved"]
+    assert roof["kernel_avg_ms"] == max(k["avg_ms"] for k in out["kernels"].values())
+    # the step pair over the whole timed run, at the line's value
+    assert abs(roof["step_pair_achieved"] - out["value"] * 304 / 1e3) < 0.01 * roof["achieved"]
+    assert roof["achievable_gbs"] == max(r["gbs"] for r in out["copy_sweep_gbs"])
+    assert abs(roof["frac_of_achievable"] - roof["achieved"] / roof["achievable_gbs"]) < 1e-3
+    assert out["adjoint_mlups"] is None and out["adjoint_sweep_mlups"] is None
     assert out["config"]["fluid_cells"] > 0
     assert out["fp16"]["mlups"] > 0 and out["fp16"]["velocity_l2_rel_err_vs_fp64"] < 1e-2
