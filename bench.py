@@ -390,20 +390,7 @@ def main():
     wall = time.perf_counter() - t0
     m1 = L64.mass_momentum()
     mlups = nfluid * args.steps / dur / 1e6
-    # per-kernel durations: the same steps launched one by one with a hipEvent
-    # after each launch on the library's stream (aa-soa: even and odd steps)
-    _, kt = L64.run_timed(args.steps)
-    kernels = {}
-    for kind, (secs, count) in kt.items():
-        if count:
-            ms_k = secs / count * 1e3
-            gbs_k = BYTES_PER_LUP["native"] * nfluid / (secs / count) / 1e9
-            kernels[kind] = {"launches": count, "avg_ms": round(ms_k, 4),
-                             "achieved_gbs": round(gbs_k, 1),
-                             "frac": round(gbs_k / HBM_PEAK_GBS, 4)}
-    dominant = max(kernels, key=lambda k: kernels[k]["avg_ms"])
-    dom = kernels[dominant]
-    log(f"kernels {kernels}; dominant {dominant}")
+
     step_s = dur / args.steps
     achieved = BYTES_PER_LUP["native"] * nfluid / step_s / 1e9
     log(f"fp64 {args.kernel}: {mlups:.0f} MLUPS, {step_s * 1e3:.3f} ms/step, "
@@ -433,6 +420,20 @@ def main():
         log(f"{form} {args.kernel}: {mlf:.0f} MLUPS ({mlf / mlups:.2f}x fp64), "
             f"u L2 err {forms[form]['velocity_l2_rel_err_vs_fp64']:.3e}")
         Lf.close()
+    # per-kernel durations: the same steps launched one by one with a hipEvent
+    # after each launch on the library's stream (aa-soa: even and odd steps)
+    _, kt = L64.run_timed(args.steps)
+    kernels = {}
+    for kind, (secs, count) in kt.items():
+        if count:
+            ms_k = secs / count * 1e3
+            gbs_k = BYTES_PER_LUP["native"] * nfluid / (secs / count) / 1e9
+            kernels[kind] = {"launches": count, "avg_ms": round(ms_k, 6),
+                             "achieved_gbs": round(gbs_k, 1),
+                             "frac": round(gbs_k / HBM_PEAK_GBS, 4)}
+    dominant = max(kernels, key=lambda k: kernels[k]["avg_ms"])
+    dom = kernels[dominant]
+    log(f"kernels {kernels}; dominant {dominant}")
     L64.close()
 
     adjoint = None if args.no_adjoint else adjoint_run(lib, lbmhip)
