@@ -326,6 +326,10 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--spheres", type=int, default=2000)
     ap.add_argument("--kernel", default="aa-soa")
+    ap.add_argument("--geometry", default=None,
+                    help="geometry name instead of spheres-<--spheres> (e.g. channel)")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the copy-shape sweep and the 19-plane pattern (profiling runs)")
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -362,17 +366,19 @@ def main():
 
     n = args.size
     dims = (n, n, n)
-    geo = f"spheres-{args.spheres}"
+    geo = args.geometry or f"spheres-{args.spheres}"
     kw = dict(omega=1.25, xforce=1e-6, body_force=True, magic=3.0 / 16.0, seed=0)
 
     copy_gbs = lib.copy_bandwidth(4 << 30, 10)
     log(f"device copy bandwidth {copy_gbs:.0f} GB/s")
-    sweep = copy_sweep(lib)
-    best_copy = max(sweep, key=lambda r: r["gbs"])
-    log(f"best copy shape {best_copy}")
-    # the AA step's loads and stores without arithmetic, on this lattice's size
-    pattern_gbs = lib.pattern_bandwidth(19, 8, True, n ** 3, 6)
-    log(f"19-plane in-place pattern bandwidth {pattern_gbs:.0f} GB/s")
+    sweep, best_copy, pattern_gbs = None, None, None
+    if not args.no_sweep:
+        sweep = copy_sweep(lib)
+        best_copy = max(sweep, key=lambda r: r["gbs"])
+        log(f"best copy shape {best_copy}")
+        # the AA step's loads and stores without arithmetic, on this lattice's size
+        pattern_gbs = lib.pattern_bandwidth(19, 8, True, n ** 3, 6)
+        log(f"19-plane in-place pattern bandwidth {pattern_gbs:.0f} GB/s")
 
     result = {}
     L64 = lbmhip.Lattice(lib, args.kernel, geo, dims, (0, 0, 1), **kw)
@@ -437,7 +443,7 @@ def main():
     L64.close()
 
     adjoint = None if args.no_adjoint else adjoint_run(lib, lbmhip)
-    traffic = traffic_from_profiles(f"{args.kernel}-fp64-{n}-{dominant}")
+    traffic = traffic_from_profiles(f"{args.kernel}-fp64-{n}-{geo}-{dominant}")
 
     cpu = None if args.no_cpu else cpu_baseline()
     cpu_omp = None if args.no_cpu else cpu_reference_openmp()
@@ -455,9 +461,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic: seeded sphere packing and seeded f_i = w_i(1+eps), built on the device",
-        "config": {"workload": f"D3Q19 TRT {args.kernel}, {n}^3 channel (walls at y) with "
-                               f"{args.spheres} seeded spheres r~U[4,12], omega 1.25, Lambda 3/16, "
-                               f"body force 1e-6", "kernel": args.kernel, "lattice": list(dims),
+        "config": {"workload": f"D3Q19 TRT {args.kernel}, {n}^3 " + (
+                               f"'{args.geometry}' geometry" if args.geometry else
+                               f"channel (walls at y) with {args.spheres} seeded spheres r~U[4,12]")
+                               + ", omega 1.25, Lambda 3/16, body force 1e-6",
+                   "kernel": args.kernel, "lattice": list(dims),
                    "fluid_cells": nfluid, "solid_fraction": round(solid_fraction, 5),
                    "storage": "fp64"},
         "roofline": {"bound": "hbm", "achieved": dom["achieved_gbs"], "peak": HBM_PEAK_GBS,
@@ -468,11 +476,13 @@ def main():
                      "traffic": traffic[0], "traffic_source": traffic[1],
                      "step_pair_achieved": round(achieved, 1),
                      "step_pair_frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "achievable_gbs": best_copy["gbs"],
+                     "achievable_gbs": best_copy["gbs"] if best_copy else None,
                      "achievable_shape": {k: best_copy[k] for k in ("vectors_per_lane",
                                                                     "non_temporal",
-                                                                    "persistent_groups")},
-                     "frac_of_achievable": round(dom["achieved_gbs"] / best_copy["gbs"], 4)},
+                                                                    "persistent_groups")}
+                     if best_copy else None,
+                     "frac_of_achievable": round(dom["achieved_gbs"] / best_copy["gbs"], 4)
+                     if best_copy else None},
         "kernels": kernels,
         "copy_sweep_gbs": sweep,
         "cpu_baseline": cpu,
@@ -485,7 +495,7 @@ def main():
         "adjoint": adjoint,
         "mass_drift_rel": (m1[0] - m0[0]) / m0[0],
         "copy_bandwidth_gbs": round(copy_gbs, 1),
-        "pattern_bandwidth_gbs": round(pattern_gbs, 1),
+        "pattern_bandwidth_gbs": round(pattern_gbs, 1) if pattern_gbs else None,
         "device": dev,
     }
     print(json.dumps(out), flush=True)

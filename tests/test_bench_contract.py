@@ -52,3 +52,23 @@ def test_bench_line_has_the_contract_fields():
     assert out["adjoint_mlups"] is None and out["adjoint_sweep_mlups"] is None
     assert out["config"]["fluid_cells"] > 0
     assert out["fp16"]["mlups"] > 0 and out["fp16"]["velocity_l2_rel_err_vs_fp64"] < 1e-2
+
+
+@pytest.mark.gpu
+def test_profiling_options():
+    """--geometry replaces the sphere packing, --kernel picks the variant and
+    --no-sweep leaves out the copy sweep and the pattern: the achievable
+    line is then null rather than invented"""
+    run = subprocess.run([sys.executable, BENCH, "--size", "64", "--geometry", "channel",
+                          "--kernel", "pull-aos", "--steps", "4", "--warmup", "2", "--no-adjoint",
+                          "--no-cpu", "--no-fp16", "--no-sweep"],
+                         capture_output=True, text=True, timeout=600)
+    assert run.returncode == 0, run.stderr[-2000:]
+    out = json.loads([ln for ln in run.stdout.splitlines() if ln.startswith('{"metric"')][0])
+    assert "'channel' geometry" in out["config"]["workload"]
+    assert out["config"]["kernel"] == "pull-aos"
+    assert out["config"]["fluid_cells"] == 64 * 62 * 64
+    roof = out["roofline"]
+    assert roof["achievable_gbs"] is None and roof["frac_of_achievable"] is None
+    assert out["copy_sweep_gbs"] is None and out["pattern_bandwidth_gbs"] is None
+    assert roof["kernel_avg_ms"] > 0 and out["fp16"] is None
