@@ -130,7 +130,7 @@ def adjoint_run(lib, lbmhip):
 def dominant_name(kernel, kind):
     """the HIP kernel behind a step kind of the bench's default settings"""
     if kernel == "aa-soa":
-        return "kRowSweep<AA odd, FP64>" if kind == "odd" else "kStepRow<AA even, FP64>"
+        return "kRowGroup<AA odd, FP64>" if kind == "odd" else "kStepRow<AA even, FP64>"
     return f"{kernel} step (FP64)"
 
 

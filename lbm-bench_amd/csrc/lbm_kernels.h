@@ -45,7 +45,8 @@ struct StepLaunch {
   int band;    // AoS row order: bands of `band` y-rows with x swept inside each (0: y fastest)
   int wgPerCu; // AoS: resident workgroups per CU, limited through the LDS allocation (0: none)
   void *post;  // push-aos in two passes: Q planes of nX nY nZ post-collision values (else null)
-  int rowSweep; // push / AA odd SoA with nZ % 64 == 0: one wavefront per row, aligned stores
+  int rowSweep; // push / AA odd SoA with nZ % 64 == 0, aligned stores: 1 one wavefront per
+                // row (kRowSweep), 3 one workgroup per row (kRowGroup, nZ <= 512)
   int dither;    // FP16/BF16 storage: stochastic rounding in the step
   // rounding seeds: step number = *seqBase + seq (seqBase in device memory,
   // advanced by every replay of a step graph; see launchBumpSeq)

@@ -813,9 +813,13 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
   D->nt      = envInt("LBM_NT", -1);
   D->wavesPerBlock = envInt("LBM_WAVES_PER_BLOCK", 4);
   D->dither        = envInt("LBM_DITHER", 1);
-  // measured at 512^3 (aa-soa): the row sweep gains 2.5 % with 8-byte and
-  // 11-13 % with 2-byte elements and loses 4 % with 4-byte elements
-  D->rowSweep      = envInt("LBM_ROW_SWEEP", D->esize == 4 ? 0 : 1);
+  // push / AA odd with nZ % 64 == 0: 1 the row sweep (one wavefront walks
+  // a row), 3 the row group (one workgroup per row, nZ <= 512; else the
+  // sweep), 0 the per-block kernel.  Measured at 512^3 spheres (aa-soa odd
+  // step, one process): 8-byte elements row group 8.40 ms, sweep 8.80,
+  // per-block 9.30; 2-byte sweep 2.32, row group 3.75; 4-byte per-block
+  // 3.98, row group 4.51, sweep 4.69
+  D->rowSweep      = envInt("LBM_ROW_SWEEP", D->esize == 8 ? 3 : (D->esize == 4 ? 0 : 1));
   D->seq           = 0;
   D->steps         = 0;
   D->dseqHost      = 0;

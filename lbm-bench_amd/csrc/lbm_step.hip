@@ -707,6 +707,178 @@ __global__ __launch_bounds__(256, LBM_MIN_WAVES) void kRowSweep(S *a, S *b,
   }
 }
 
+// ---- row group: one workgroup per row, one wavefront per 64-block ------------
+// The row sweep's line-aligned z-shifted loads and stores without its serial
+// walk.  The nZ/64 wavefronts of a workgroup take the blocks of one (x, y)
+// row together; the one element per block and direction that a lane shift
+// takes from the neighbouring block (lane 63 of block k - 1 for c_z = +1,
+// lane 0 of block k + 1 for c_z = -1, wrapping in z) comes through LDS.
+// A row's blocks then reach HBM together, as in kStepRow, instead of one
+// block after the other.  AA odd (in place): every load of the workgroup
+// completes before its first barrier and every store follows its second,
+// so the slots that a lane shift moves between wavefronts are read before
+// they are written (each slot is read and written by the one cell that
+// owns it).  Results equal the row sweep's bit for bit.
+template <int V, int NT, typename S>
+__global__ __launch_bounds__(512) void kRowGroup(S *a, S *b, const MaskWin *__restrict__ mask,
+    Params<S> p, int tiled)
+{
+  using R  = typename Store<S>::R;
+  using St = Store<S>;
+  constexpr bool NTS = NT & 1, NTL = NT & 2;
+  static_assert(V == V_PUSH || V == V_AA_ODD, "row group: push and AA odd only");
+  constexpr int E0 = T; // the ten directions with c_z != 0 are T .. BE
+  __shared__ S xg[8][10], xs[8][10];
+
+  int row;
+  if (tiled) {
+    // the rows of a 2 x 2 tile are consecutive workgroups (y faster)
+    const unsigned t = blockIdx.x >> 2, i = blockIdx.x & 3u;
+    const unsigned tY = (unsigned)(p.ny + 1) / 2u;
+    const int xr = (int)(t / tY) * 2 + (int)(i >> 1), yr = (int)(t % tY) * 2 + (int)(i & 1u);
+    if (xr >= p.nx || yr >= p.ny)
+      return; // the whole workgroup: no barrier is left waiting
+    row = xr * p.ny + yr;
+  } else {
+    row = (int)blockIdx.x;
+  }
+  const int w    = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = (int)(threadIdx.x & 63u);
+  const int nz = p.nz, K = p.segs, k = w;
+  const int wl = w == 0 ? K - 1 : w - 1, wr = w == K - 1 ? 0 : w + 1;
+  const int x = row / p.ny, y = row - x * p.ny;
+  const int xs_[3] = { wrapDec(x, p.nx), x, wrapInc(x, p.nx) };
+  const int ys_[3] = { wrapDec(y, p.ny), y, wrapInc(y, p.ny) };
+  auto rowOf = [&](int ox, int oy) -> uint64_t {
+    return (uint64_t)xs_[ox + 1] * (uint64_t)p.ny + (uint64_t)ys_[oy + 1];
+  };
+  // slot offsets from a stride the compiler cannot see through (kept live,
+  // the distinct 64-bit offsets would exceed the SGPRs)
+  auto sl = [&](uint64_t c, int d) -> uint64_t {
+    return c + (uint64_t)(uint32_t)d * (uint64_t)opaqueS((uint32_t)p.plane);
+  };
+  auto rowStart = [&](int ox, int oy) -> uint64_t {
+    const uint32_t r = (uint32_t)xs_[ox + 1] * (uint32_t)p.ny + (uint32_t)ys_[oy + 1];
+    return (uint64_t)(r * opaqueS((uint32_t)nz));
+  };
+  auto bitsOf = [&](int ox, int oy, int oz) -> uint64_t {
+    const RowBits rb = rowBits(mask[rowOf(ox, oy) * K + k]);
+    return oz < 0 ? rb.m : (oz > 0 ? rb.p : rb.c);
+  };
+  S *o = V == V_PUSH ? b : a;
+
+  const int z         = k * 64 + lane;
+  const uint64_t cell = rowStart(0, 0) + (uint64_t)z;
+  const MaskWin wc    = mask[rowOf(0, 0) * K + k];
+  // clean: no cell of the 3x3 rows around the block (z halo included) is
+  // solid, so there is no bounce-back and no store to skip
+  const bool clean = !nearSolid(wc);
+  const bool fluid = clean || !__builtin_amdgcn_inverse_ballot_w64(wc.lo);
+
+  // loads: AA odd gathers from x - c_d, aligned (element z of the
+  // neighbour row, shifted below); push loads its own cell
+  S al[Q], own[Q];
+  auto gather = [&](auto walls) {
+    constexpr bool W = decltype(walls)::value;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+      own[d] = S{};
+      if constexpr (V == V_AA_ODD) {
+        al[d] = getS<NTL>(&a[sl(rowStart(-cx(d), -cy(d)) + (uint64_t)z, inv(d))]);
+        if (W && d != C && __builtin_amdgcn_inverse_ballot_w64(bitsOf(-cx(d), -cy(d), -cz(d))))
+          own[d] = getS<NTL>(&a[sl(cell, d)]); // bounce-back: own slot d
+      } else {
+        al[d] = fluid ? getS<NTL>(&a[sl(cell, d)]) : S{};
+      }
+    }
+  };
+  if (clean)
+    gather(std::false_type{});
+  else
+    gather(std::true_type{});
+  if constexpr (V == V_AA_ODD) {
+#pragma unroll
+    for (int d = E0; d < E0 + 10; ++d) {
+      if ((cz(d) > 0 && lane == 63) || (cz(d) < 0 && lane == 0))
+        xg[w][d - E0] = al[d];
+    }
+  }
+  __syncthreads();
+
+  R f[Q];
+  auto take = [&](auto walls) {
+    constexpr bool W = decltype(walls)::value;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+      S v = al[d];
+      if constexpr (V == V_AA_ODD) {
+        if (cz(d) != 0)
+          v = laneShift(al[d], cz(d) > 0 ? xg[wl][d - E0] : xg[wr][d - E0], cz(d) > 0);
+        if (W && d != C && __builtin_amdgcn_inverse_ballot_w64(bitsOf(-cx(d), -cy(d), -cz(d))))
+          v = own[d];
+      }
+      f[d] = St::raw(v);
+    }
+  };
+  if (clean)
+    take(std::false_type{});
+  else
+    take(std::true_type{});
+  if (p.addForce)
+    addForce(f, p.f1, p.f2);
+  collide(f, p.trt);
+  const Dither dz = ditherOf(cell, stepSeq(p), p.dither);
+  S v[Q];
+#pragma unroll
+  for (int d = 0; d < Q; ++d) {
+    v[d] = St::rawStep(f[d], ditherDir(dz, d));
+    if ((cz(d) > 0 && lane == 63) || (cz(d) < 0 && lane == 0))
+      xs[w][d - E0] = v[d];
+  }
+  __syncthreads();
+
+  // stores to x + c_d; the z-shifted ones line-aligned: lane l of block k
+  // writes element 64k + l with the value of cell 64k + l - c_z, skipped
+  // where that producer is solid (its fluid target fills the slot by
+  // bounce-back)
+  auto scatter = [&](auto walls) {
+    constexpr bool W = decltype(walls)::value;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+      if (d == C) {
+        if (fluid)
+          putS<NTS>(&o[sl(cell, C)], v[C]);
+        continue;
+      }
+      if (W && fluid && __builtin_amdgcn_inverse_ballot_w64(bitsOf(cx(d), cy(d), cz(d))))
+        putS<NTS>(&o[sl(cell, inv(d))], v[d]); // bounce-back: own slot inv d
+      const uint64_t tgt = rowStart(cx(d), cy(d)) + (uint64_t)z;
+      if (cz(d) == 0) {
+        if (fluid)
+          putS<NTS>(&o[sl(tgt, d)], v[d]);
+        continue;
+      }
+      const S sh = laneShift(v[d], cz(d) > 0 ? xs[wl][d - E0] : xs[wr][d - E0], cz(d) > 0);
+      const uint64_t skip = W ? bitsOf(0, 0, -cz(d)) : 0ull;
+      if (!__builtin_amdgcn_inverse_ballot_w64(skip))
+        putS<NTS>(&o[sl(tgt, d)], sh);
+    }
+  };
+  if (clean)
+    scatter(std::false_type{});
+  else
+    scatter(std::true_type{});
+}
+
+template <int V, int NT, typename S>
+static void launchRowGroup(const StepLaunch &L, hipStream_t s)
+{
+  Params<S> p           = makeParams<S>(L);
+  const unsigned blocks = L.tiled ? tileBlocks(L.g, 1) * 4u : (unsigned)((uint64_t)L.g.nx * L.g.ny);
+  hipLaunchKernelGGL((kRowGroup<V, NT, S>), dim3(blocks), dim3(64 * L.g.segs), 0, s, (S *)L.src,
+      (S *)L.dst, L.g.mask, p, L.tiled);
+}
+
 template <int V, int NT, typename S>
 static void launchRowSweep(const StepLaunch &L, hipStream_t s)
 {
@@ -723,8 +895,11 @@ static void launchT(const StepLaunch &L, hipStream_t s)
     launchAoS<V, NT, S>(L, s);
   } else {
     if constexpr (V == V_PUSH || V == V_AA_ODD) {
-      if (L.rowSweep && L.g.nz % 64 == 0 && L.g.plane < (1ull << 32))
+      if (L.rowSweep && L.g.nz % 64 == 0 && L.g.plane < (1ull << 32)) {
+        if (L.rowSweep == 3 && L.g.segs <= 8)
+          return launchRowGroup<V, NT, S>(L, s);
         return launchRowSweep<V, NT, S>(L, s);
+      }
     }
     Params<S> p       = makeParams<S>(L);
     const int wpb     = L.tiled ? 4 : (L.wavesPerBlock == 8 ? 8 : 4);

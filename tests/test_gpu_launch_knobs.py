@@ -8,7 +8,7 @@ stochastic rounding seeds depend only on cell and step.  So each setting
 must give the default run's state bit for bit, for FP64 and for FP16
 storage, on a sphere channel with several 64-cell blocks per row (walls,
 solids inside the rows, the z wrap) and the body force on.
-LBM_ROW_SWEEP (kRowSweep / kStepRow) and LBM_AOS_PUSH_PASSES (kStepAoS /
+LBM_ROW_SWEEP (kRowGroup / kRowSweep / kStepRow) and LBM_AOS_PUSH_PASSES (kStepAoS /
 kPushCollideAoS) swap kernels, whose compilations fuse multiply-adds of
 the collision differently: there the bound is 1e-14 relative (FP64) or a
 few FP16 quanta of the stored deviation (FP16)."""
@@ -29,6 +29,7 @@ KNOBS = [
     {"LBM_NT": "3"},
     {"LBM_ROW_SWEEP": "0"},
     {"LBM_ROW_SWEEP": "1"},
+    {"LBM_ROW_SWEEP": "3"},
     {"LBM_WAVES_PER_BLOCK": "8"},
     {"LBM_PLANE_PAD": "4096"},
     {"LBM_GRAPH_STEPS": "4"},

@@ -56,6 +56,10 @@ CASES = [
     ("spheres-6", (16, 18, 128), (0, 0, 1)),
     ("box", (10, 11, 192), (1, 0, 1)),
     ("fluid", (6, 7, 192), (1, 1, 1)),
+    # the row group at its largest workgroup (eight 64-blocks per row), and
+    # nZ = 576, past it, where the row sweep runs instead
+    ("spheres-3", (6, 8, 512), (0, 0, 1)),
+    ("spheres-3", (4, 6, 576), (0, 0, 1)),
 ]
 
 
