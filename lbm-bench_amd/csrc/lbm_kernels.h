@@ -48,6 +48,8 @@ struct StepLaunch {
   int rowSweep; // push / AA odd SoA with nZ % 64 == 0, aligned stores: 1 one wavefront per
                 // row (kRowSweep), 3 one workgroup per row (kRowGroup, nZ <= 512)
   int dither;    // FP16/BF16 storage: stochastic rounding in the step
+  int wallNt;    // kRowGroup AA odd, 8-byte, nt 3: non-temporal bits of the blocks near a
+                 // solid, stores (1) / aligned gathers (2) / bounce-back loads (4); -1: 1
   // rounding seeds: step number = *seqBase + seq (seqBase in device memory,
   // advanced by every replay of a step graph; see launchBumpSeq)
   const uint32_t *seqBase;

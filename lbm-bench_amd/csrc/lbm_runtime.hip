@@ -84,7 +84,7 @@ struct LbmDevice {
   float scale, invScale;
   double *dpart, *dout;
   int npart;
-  int remap, nt, wavesPerBlock, dither, rowSweep, band, wgPerCu;
+  int remap, nt, wavesPerBlock, dither, rowSweep, band, wgPerCu, wallNt;
   // FP16/BF16 rounding seeds.  Every step is seeded with its absolute
   // number, steps, however the steps are grouped into graph replays and
   // eager launches, so the rounding does not depend on LBM_GRAPH_STEPS.
@@ -381,6 +381,7 @@ static StepLaunch baseLaunch(const LbmDevice *D, const CaseDataType *cd)
   L.wavesPerBlock = D->wavesPerBlock;
   L.rowSweep      = D->rowSweep;
   L.dither        = D->dither;
+  L.wallNt        = D->wallNt;
   return L;
 }
 
@@ -813,6 +814,9 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
   D->nt      = envInt("LBM_NT", -1);
   D->wavesPerBlock = envInt("LBM_WAVES_PER_BLOCK", 4);
   D->dither        = envInt("LBM_DITHER", 1);
+  // kRowGroup (AA odd, FP64): non-temporal bits of the blocks near a
+  // solid (LBM_WALL_NT 0-7; unset: cached loads, non-temporal stores)
+  D->wallNt        = envInt("LBM_WALL_NT", -1);
   // push / AA odd with nZ % 64 == 0: 1 the row sweep (one wavefront walks
   // a row), 3 the row group (one workgroup per row, nZ <= 512; else the
   // sweep), 0 the per-block kernel.  Measured at 512^3 spheres (aa-soa odd

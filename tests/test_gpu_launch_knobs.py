@@ -1,7 +1,8 @@
 """Launch shape does not change results.
 
 Every launch knob (block order across XCDs, row order in bands, workgroups
-per CU, push-aos in one pass or two, non-temporal hints, the row sweep
+per CU, push-aos in one pass or two, non-temporal hints (also those of
+the row group's blocks near a solid), the row sweep
 against the per-block kernel, wavefronts per workgroup, plane padding)
 picks another way to run the same arithmetic on the same slots, and the
 stochastic rounding seeds depend only on cell and step.  So each setting
@@ -30,6 +31,8 @@ KNOBS = [
     {"LBM_ROW_SWEEP": "0"},
     {"LBM_ROW_SWEEP": "1"},
     {"LBM_ROW_SWEEP": "3"},
+    {"LBM_WALL_NT": "0"},
+    {"LBM_WALL_NT": "7"},
     {"LBM_WAVES_PER_BLOCK": "8"},
     {"LBM_PLANE_PAD": "4096"},
     {"LBM_GRAPH_STEPS": "4"},
