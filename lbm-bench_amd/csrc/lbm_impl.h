@@ -12,6 +12,8 @@ struct Params {
   Trt<R> trt;
   R f1, f2;       // body force terms, added to f_E,f_NE,... and subtracted
                   // from f_W,f_NW,... (3 w c_x g of src/lbm.c:919)
+  R cf1, cf2;     // the same terms as the force pass forms them (the double
+                  // values converted to R), added at the stores of a closing step
   R scale, inv;   // storage scale and its inverse (exact powers of two)
   uint64_t plane;
   int nx, ny, nz, segs, nrows;
@@ -51,6 +53,8 @@ inline Params<S> makeParams(const StepLaunch &L)
   p.segs     = L.g.segs;
   p.nrows    = L.g.nx * L.g.ny;
   p.addForce = L.addForce;
+  p.cf1      = (R)L.f1;
+  p.cf2      = (R)L.f2;
   p.dither   = L.dither;
   p.seqBase  = L.seqBase;
   p.seq      = L.seq;
@@ -62,6 +66,8 @@ inline Params<S> makeParams(const StepLaunch &L)
     p.trt.d15 *= p.inv;
     p.f1 *= p.scale;
     p.f2 *= p.scale;
+    p.cf1 *= p.scale;
+    p.cf2 *= p.scale;
   }
   return p;
 }
@@ -119,6 +125,28 @@ __device__ __forceinline__ void storeRow(S *o, uint64_t plane, uint64_t rowCell,
 #pragma unroll
   for (int d = 0; d < Q; ++d)
     putS<NTS>(rowSlot(o, INV ? inv(d) : d, plane, rowCell, z), Store<S>::rawStep(f[d], ditherDir(dz, d)));
+}
+
+// Closing steps.  The value a push or AA odd step streams to x + c_d is
+// the logical f_d there after the step, and the bounce-back value it
+// stores at x is the logical f_inv d of x.  The last step of a run can
+// therefore add its own body force at the store, +t_d to a streamed value
+// and -t_d to a bounce-back one, as the force pass after the step would
+// (src/lbm.c:902-922): the force pass loads the stored value and adds the
+// term with one rounding, which is the same arithmetic when the stored
+// value is the collided one, i.e. storage without stochastic rounding.
+template <typename S>
+constexpr bool closable()
+{
+  return std::is_same<S, double>::value || std::is_same<S, float>::value ||
+         std::is_same<S, f32dev>::value;
+}
+// t_d for a direction with c_x != 0
+template <typename R>
+__device__ __forceinline__ R closeTerm(int d, R cf1, R cf2)
+{
+  const R fd = wclass(d) == 1 ? cf1 : cf2;
+  return cx(d) > 0 ? fd : -fd;
 }
 
 // body force on the 10 directions with c_x != 0

@@ -39,6 +39,10 @@ struct StepLaunch {
   float f1f, f2f;
   float scale, invScale;
   int addForce;
+  // the run's last step (push or AA odd, storage without stochastic
+  // rounding): add this step's own body force at the store, so that no
+  // closing force pass is needed
+  int close;
   // launch shape: XCD-aware block order, non-temporal stores (bit 0) /
   // loads (bit 1), wavefronts per workgroup (4 or 8)
   int remap, nt, wavesPerBlock;

@@ -84,7 +84,7 @@ struct LbmDevice {
   float scale, invScale;
   double *dpart, *dout;
   int npart;
-  int remap, nt, wavesPerBlock, dither, rowSweep, band, wgPerCu, wallNt;
+  int remap, nt, wavesPerBlock, dither, rowSweep, band, wgPerCu, wallNt, closeFused;
   // FP16/BF16 rounding seeds.  Every step is seeded with its absolute
   // number, steps, however the steps are grouped into graph replays and
   // eager launches, so the rounding does not depend on LBM_GRAPH_STEPS.
@@ -404,10 +404,11 @@ static void replaySteps(LbmDevice *D, hipGraphExec_t g, int G)
 // one step; for two-lattice variants src is D->d[srcIdx].  node >= 0 is the
 // step's index inside a graph being captured, -1 an eager launch.
 static void launchStep(LbmDevice *D, StepLaunch L, int variant, int srcIdx, int addForce,
-    hipStream_t s, int node = -1)
+    hipStream_t s, int node = -1, int close = 0)
 {
   L.variant  = variant;
   L.addForce = addForce;
+  L.close    = close;
   // XCD-aware block order: on for the AoS layout, whose scattered side
   // collects each record line from nine neighbour rows, so rows that share
   // lines should share an L2 (measured at 256^3: pull-aos +66 %, push-aos
@@ -624,36 +625,44 @@ static void runKernel(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cdI
     D->tev.assign(1, e);
     D->tkind.assign(1, -1);
   }
+  // The last step adds its own force at the store where it can (push and
+  // AA odd on SoA storage without stochastic rounding, no pressure
+  // boundaries after it): the closing force pass then drops out.  That
+  // step is launched directly, after the graph replays.
+  const bool fuse = bf && !D->zhOn && D->layout == LAYOUT_SOA &&
+                    (D->prop == PROP_AA || D->prop == PROP_PUSH) &&
+                    (D->storage == ST_F64 || D->storage == ST_F32 || D->storage == ST_F32DEV) &&
+                    D->closeFused;
   if (n > 0) {
     int left = n;
     if (D->prop == PROP_AA) {
       launchStep(D, base, V_AA_EVEN, 0, 0, D->stream);
-      launchStep(D, base, V_AA_ODD, 0, bf, D->stream);
+      launchStep(D, base, V_AA_ODD, 0, bf, D->stream, -1, fuse && left == 2);
       left -= 2;
-      while (G > 0 && left >= G) {
+      while (G > 0 && left >= G + (fuse ? 2 : 0)) {
         replaySteps(D, D->gexec[0], G);
         left -= G;
       }
       while (left > 0) {
         launchStep(D, base, V_AA_EVEN, 0, bf, D->stream);
-        launchStep(D, base, V_AA_ODD, 0, bf, D->stream);
+        launchStep(D, base, V_AA_ODD, 0, bf, D->stream, -1, fuse && left == 2);
         left -= 2;
       }
     } else {
-      launchStep(D, base, stepVariant(D), D->active, 0, D->stream);
+      launchStep(D, base, stepVariant(D), D->active, 0, D->stream, -1, fuse && left == 1);
       D->active ^= 1;
       left -= 1;
-      while (G > 0 && left >= G) {
+      while (G > 0 && left >= G + (fuse ? 1 : 0)) {
         replaySteps(D, D->gexec[D->active], G);
         left -= G; // G is even: the state ends in the same array
       }
       while (left > 0) {
-        launchStep(D, base, stepVariant(D), D->active, bf, D->stream);
+        launchStep(D, base, stepVariant(D), D->active, bf, D->stream, -1, fuse && left == 1);
         D->active ^= 1;
         left -= 1;
       }
     }
-    if (bf) {
+    if (bf && !fuse) {
       const int rule = D->prop == PROP_PULL ? SR_PULL : SR_LOCAL;
       AuxLaunch A    = auxOf(D, rule);
       A.seq          = forceSeq(D);
@@ -817,6 +826,9 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
   // kRowGroup (AA odd, FP64): non-temporal bits of the blocks near a
   // solid (LBM_WALL_NT 0-7; unset: cached loads, non-temporal stores)
   D->wallNt        = envInt("LBM_WALL_NT", -1);
+  // the last step of a run adds its own body force at the store
+  // (LBM_CLOSE_FUSED 0: a separate force pass after it, as before)
+  D->closeFused    = envInt("LBM_CLOSE_FUSED", 1);
   // push / AA odd with nZ % 64 == 0: 1 the row sweep (one wavefront walks
   // a row), 3 the row group (one workgroup per row, nZ <= 512; else the
   // sweep), 0 the per-block kernel.  Measured at 512^3 spheres (aa-soa odd

@@ -42,10 +42,11 @@ namespace lbm {
 #define LBM_MIN_WAVES 1
 #endif
 
-template <int V, int NT, typename S>
+template <int V, int NT, typename S, bool CL = false>
 __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const MaskWin *__restrict__ mask,
     Params<S> p, int remap, int tiled)
 {
+  static_assert(!CL || ((V == V_PUSH || V == V_AA_ODD) && closable<S>()), "closing: push / AA odd");
   using R = typename Store<S>::R;
   using St = Store<S>;
   constexpr bool NTS = NT & 1, NTL = NT & 2;
@@ -115,13 +116,20 @@ __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
       S *ptr;
+      R fv = f[d];
       if (d == C) {
         ptr = rowSlot(o, C, p.plane, c.rowCell, c.z);
       } else {
         const bool bb = solidOf(cx(d), cy(d), cz(d));
         ptr = bb ? slotPtr(o, inv(d), p.plane, cell) : slotPtr(o, d, p.plane, cellOf(cx(d), cy(d), cz(d)));
+        if constexpr (CL) {
+          if (cx(d) != 0) {
+            const R t = closeTerm(d, p.cf1, p.cf2);
+            fv += bb ? -t : t;
+          }
+        }
       }
-      putS<NTS>(ptr, St::rawStep(f[d], ditherDir(dz, d)));
+      putS<NTS>(ptr, St::rawStep(fv, ditherDir(dz, d)));
     }
   }
 }
@@ -480,10 +488,11 @@ __device__ __forceinline__ S laneValue(S v, int lane)
   }
 }
 
-template <int V, int NT, typename S>
+template <int V, int NT, typename S, bool CL = false>
 __global__ __launch_bounds__(256, LBM_MIN_WAVES) void kRowSweep(S *a, S *b,
     const MaskWin *__restrict__ mask, Params<S> p, int remap, int tiled)
 {
+  static_assert(!CL || closable<S>(), "closing: storage without stochastic rounding");
   using R  = typename Store<S>::R;
   using St = Store<S>;
   constexpr bool NTS = NT & 1, NTL = NT & 2;
@@ -642,7 +651,19 @@ __global__ __launch_bounds__(256, LBM_MIN_WAVES) void kRowSweep(S *a, S *b,
 
 #pragma unroll
       for (int d = 0; d < Q; ++d) {
-        const S v = St::rawStep(f[d], ditherDir(dz, d));
+        // streamed value v; bounce-back value vb (they differ in a closing step)
+        S v, vb;
+        if constexpr (CL) {
+          if (cx(d) != 0) {
+            const R t = closeTerm(d, p.cf1, p.cf2);
+            v         = St::rawStep(f[d] + t, ditherDir(dz, d));
+            vb        = St::rawStep(f[d] - t, ditherDir(dz, d));
+          } else {
+            v = vb = St::rawStep(f[d], ditherDir(dz, d));
+          }
+        } else {
+          v = vb = St::rawStep(f[d], ditherDir(dz, d));
+        }
         if (d == C) {
           if (fluid)
             putS<NTS>(&o[sl(cell, C)], v);
@@ -652,11 +673,11 @@ __global__ __launch_bounds__(256, LBM_MIN_WAVES) void kRowSweep(S *a, S *b,
           if (fluid)
             putS<NTS>(&o[sl(cellN(cx(d), cy(d), 0), d)], v);
           if (W && fluid && __builtin_amdgcn_inverse_ballot_w64(bitsOf(cx(d), cy(d), 0, k)))
-            putS<NTS>(&o[sl(cell, inv(d))], v); // bounce-back: own slot inv d
+            putS<NTS>(&o[sl(cell, inv(d))], vb); // bounce-back: own slot inv d
           continue;
         }
         if (W && fluid && __builtin_amdgcn_inverse_ballot_w64(bitsOf(cx(d), cy(d), cz(d), k)))
-          putS<NTS>(&o[sl(cell, inv(d))], v); // bounce-back
+          putS<NTS>(&o[sl(cell, inv(d))], vb); // bounce-back
         const uint64_t tgt = rowStart(cx(d), cy(d));
         if (cz(d) > 0) {
           const S sh = laneShift(v, carry[d], true);
@@ -719,10 +740,11 @@ __global__ __launch_bounds__(256, LBM_MIN_WAVES) void kRowSweep(S *a, S *b,
 // so the slots that a lane shift moves between wavefronts are read before
 // they are written (each slot is read and written by the one cell that
 // owns it).  Results equal the row sweep's bit for bit.
-template <int V, int NT, typename S, int WNT = (NT | ((NT & 2) << 1))>
+template <int V, int NT, typename S, int WNT = (NT | ((NT & 2) << 1)), bool CL = false>
 __global__ __launch_bounds__(512) void kRowGroup(S *a, S *b, const MaskWin *__restrict__ mask,
     Params<S> p, int tiled, int remap)
 {
+  static_assert(!CL || closable<S>(), "closing: storage without stochastic rounding");
   using R  = typename Store<S>::R;
   using St = Store<S>;
   constexpr bool NTS = NT & 1, NTL = NT & 2;
@@ -837,14 +859,35 @@ __global__ __launch_bounds__(512) void kRowGroup(S *a, S *b, const MaskWin *__re
   if (p.addForce)
     addForce(f, p.f1, p.f2);
   collide(f, p.trt);
+  // the collided values as they are: the closing compilation uses them
+  // twice (streamed and bounce-back values), which would otherwise let the
+  // compiler fuse the collision's multiply-adds differently from the plain
+  // compilation (last-bit differences between a run closed by this kernel
+  // and one continued past the same step)
+#pragma unroll
+  for (int d = 0; d < Q; ++d)
+    asm("" : "+v"(f[d]));
   const Dither dz = ditherOf(cell, stepSeq(p), p.dither);
   S v[Q];
 #pragma unroll
   for (int d = 0; d < Q; ++d) {
-    v[d] = St::rawStep(f[d], ditherDir(dz, d));
+    R fv = f[d];
+    if constexpr (CL) {
+      if (cx(d) != 0)
+        fv += closeTerm(d, p.cf1, p.cf2);
+    }
+    v[d] = St::rawStep(fv, ditherDir(dz, d));
     if ((cz(d) > 0 && lane == 63) || (cz(d) < 0 && lane == 0))
       xs[w][d - E0] = v[d];
   }
+  // bounce-back value of direction d (it differs from v[d] in a closing step)
+  auto bbValue = [&](int d) -> S {
+    if constexpr (CL) {
+      if (cx(d) != 0)
+        return St::rawStep(f[d] - closeTerm(d, p.cf1, p.cf2), ditherDir(dz, d));
+    }
+    return v[d];
+  };
   __syncthreads();
 
   // stores to x + c_d; the z-shifted ones line-aligned: lane l of block k
@@ -862,7 +905,7 @@ __global__ __launch_bounds__(512) void kRowGroup(S *a, S *b, const MaskWin *__re
         continue;
       }
       if (W && fluid && __builtin_amdgcn_inverse_ballot_w64(bitsOf(cx(d), cy(d), cz(d))))
-        putS<S_>(&o[sl(cell, inv(d))], v[d]); // bounce-back: own slot inv d
+        putS<S_>(&o[sl(cell, inv(d))], bbValue(d)); // bounce-back: own slot inv d
       const uint64_t tgt = rowStart(cx(d), cy(d)) + (uint64_t)z;
       if (cz(d) == 0) {
         if (fluid)
@@ -889,6 +932,17 @@ static void launchRowGroup(const StepLaunch &L, hipStream_t s)
 #define LAUNCH_RG(W)                                                                         \
   hipLaunchKernelGGL((kRowGroup<V, NT, S, W>), dim3(blocks), dim3(64 * L.g.segs), 0, s,     \
       (S *)L.src, (S *)L.dst, L.g.mask, p, L.tiled, L.remap)
+  if constexpr (closable<S>()) {
+    if (L.close) {
+      // the closing step: one compilation per variant and storage, with the
+      // default non-temporal bits (they do not change results)
+      constexpr int NTC = sizeof(S) == 8 ? 3 : 1;
+      constexpr int WC  = V == V_AA_ODD && sizeof(S) == 8 ? 1 : (NTC | ((NTC & 2) << 1));
+      hipLaunchKernelGGL((kRowGroup<V, NTC, S, WC, true>), dim3(blocks), dim3(64 * L.g.segs), 0,
+          s, (S *)L.src, (S *)L.dst, L.g.mask, p, L.tiled, L.remap);
+      return;
+    }
+  }
   if constexpr (V == V_AA_ODD && sizeof(S) == 8 && NT == 3) {
     switch (L.wallNt) {
     case 0: LAUNCH_RG(0); return;
@@ -911,6 +965,14 @@ static void launchRowSweep(const StepLaunch &L, hipStream_t s)
 {
   Params<S> p           = makeParams<S>(L);
   const unsigned blocks = L.tiled ? tileBlocks(L.g, 1) : (unsigned)(((uint64_t)L.g.nx * L.g.ny + 3) / 4);
+  if constexpr (closable<S>()) {
+    if (L.close) {
+      constexpr int NTC = sizeof(S) == 8 ? 3 : 1;
+      hipLaunchKernelGGL((kRowSweep<V, NTC, S, true>), dim3(blocks), dim3(256), 0, s, (S *)L.src,
+          (S *)L.dst, L.g.mask, p, L.remap, L.tiled);
+      return;
+    }
+  }
   hipLaunchKernelGGL((kRowSweep<V, NT, S>), dim3(blocks), dim3(256), 0, s, (S *)L.src,
       (S *)L.dst, L.g.mask, p, L.remap, L.tiled);
 }
@@ -931,6 +993,14 @@ static void launchT(const StepLaunch &L, hipStream_t s)
     Params<S> p       = makeParams<S>(L);
     const int wpb     = L.tiled ? 4 : (L.wavesPerBlock == 8 ? 8 : 4);
     const unsigned blocks = L.tiled ? tileBlocks(L.g, L.g.segs) : rowBlocks(L.g, wpb);
+    if constexpr ((V == V_PUSH || V == V_AA_ODD) && closable<S>()) {
+      if (L.close) {
+        constexpr int NTC = sizeof(S) == 8 ? 3 : 1;
+        hipLaunchKernelGGL((kStepRow<V, NTC, S, true>), dim3(blocks), dim3(64 * wpb), 0, s,
+            (S *)L.src, (S *)L.dst, L.g.mask, p, L.remap, L.tiled);
+        return;
+      }
+    }
     hipLaunchKernelGGL((kStepRow<V, NT, S>), dim3(blocks), dim3(64 * wpb), 0, s, (S *)L.src,
         (S *)L.dst, L.g.mask, p, L.remap, L.tiled);
   }

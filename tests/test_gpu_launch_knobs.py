@@ -74,3 +74,34 @@ def test_knobs_leave_results_bit_identical(gpu_lib, monkeypatch, kernel, storage
             continue
         same = np.array_equal(got.view(np.uint64), ref.view(np.uint64))
         assert same, (kernel, storage, knob, float(np.abs(got - ref).max()))
+
+
+def _state_fluid(lib, kernel, storage, steps):
+    with Lattice(lib, kernel, "spheres-6", DIMS, (0, 0, 1), omega=1.25, xforce=1e-5,
+                 body_force=True, magic=3 / 16, storage=storage) as L:
+        L.init("random", 9, 1.0, 1e-3)
+        L.run(steps)
+        return L.get_nodes(), L.fluid_mask()
+
+
+@pytest.mark.parametrize("precision,storage", [("dp", "native"), ("dp", "fp32"), ("sp", "native")])
+@pytest.mark.parametrize("kernel", ["aa-soa", "push-soa", "blk-push-soa"])
+def test_closing_step_equals_force_pass(request, monkeypatch, precision, storage, kernel):
+    """The last step of a run adds its own body force at the store
+    (LBM_CLOSE_FUSED, default on) instead of a force pass after it: the
+    fluid cells' PDFs are the same bit for bit, after runs of one step, an
+    even and odd number of steps, and more steps than a step graph holds;
+    the row group, row sweep and per-block kernels each close a run"""
+    lib = request.getfixturevalue("gpu_lib" if precision == "dp" else "gpu_lib_sp")
+    for steps in (1, 2, 7, 40):
+        for sweep in ("3", "1", "0"):
+            with monkeypatch.context() as m:
+                m.setenv("LBM_ROW_SWEEP", sweep)
+                fused, fluid = _state_fluid(lib, kernel, storage, steps)
+                m.setenv("LBM_CLOSE_FUSED", "0")
+                ref, _ = _state_fluid(lib, kernel, storage, steps)
+            assert np.isfinite(ref[fluid]).all()
+            w = np.uint64 if ref.dtype == np.float64 else np.uint32
+            same = np.array_equal(fused[fluid].view(w), ref[fluid].view(w))
+            assert same, (kernel, storage, precision, steps, sweep,
+                          float(np.abs(fused - ref)[fluid].max()))
