@@ -333,6 +333,9 @@ def main():
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--study-steps", type=int, default=None,
+                    help="steps per variant of --study variants256 (default 500; fewer for "
+                         "counter runs)")
     ap.add_argument("--study", choices=["precision5000", "adjoint384", "variants256", "ceiling"],
                     help="instead of the bench line, one JSON line of a longer study: the four "
                          "storage forms over 5000 steps, the 384^3 FP16/FP32 adjoint with "
@@ -355,7 +358,8 @@ def main():
               flush=True)
         return
     if args.study == "variants256":
-        print(json.dumps(study_variants(lib, lbmhip, 256, 500, cpu=not args.no_cpu)), flush=True)
+        print(json.dumps(study_variants(lib, lbmhip, 256, args.study_steps or 500,
+                                        cpu=not args.no_cpu)), flush=True)
         return
     if args.study == "ceiling":
         print(json.dumps(study_ceiling(lib, args.size)), flush=True)
