@@ -386,6 +386,9 @@ def main():
 
     result = {}
     L64 = lbmhip.Lattice(lib, args.kernel, geo, dims, (0, 0, 1), **kw)
+    placement = L64.placement()
+    log(f"placement probe: {placement[0]} candidate allocation(s), kept one at "
+        f"{placement[1]:.0f} GB/s")
     nfluid = L64.n_fluid
     solid_fraction = 1.0 - nfluid / float(n ** 3)
     log(f"lattice {dims}, {nfluid} fluid cells, solid fraction {solid_fraction:.4f}, "
@@ -498,6 +501,7 @@ def main():
         "adjoint_sweep_mlups": adjoint["fp64"]["mlups_adjoint_sweep"] if adjoint else None,
         "adjoint": adjoint,
         "mass_drift_rel": (m1[0] - m0[0]) / m0[0],
+        "placement_probe": {"candidates": placement[0], "kept_gbs": round(placement[1], 1)},
         "copy_bandwidth_gbs": round(copy_gbs, 1),
         "pattern_bandwidth_gbs": round(pattern_gbs, 1) if pattern_gbs else None,
         "device": dev,

@@ -121,6 +121,12 @@ double lbmRunStepsTimed(LatticeDescType *ld, KernelDataType *kd, CaseDataType *c
  * post-collision planes when it runs in two passes) */
 size_t lbmDeviceBytes(KernelDataType *kd);
 
+/* How the PDF arrays were placed: *candidates allocations were probed with
+ * the step's 19-stream pattern and the fastest kept, at *gbs GB/s (1 and
+ * 0.0 when no probe ran: arrays below 4 GiB, AoS, LBM_PLACEMENT_PROBE < 2).
+ * No counterpart in the reference. */
+void lbmPlacement(KernelDataType *kd, int *candidates, double *gbs);
+
 /* Plain device-to-device copy of `bytes` bytes, repeated `reps` times on one
  * stream; returns GB/s (read + write bytes / second) from hipEvents.  The
  * achievable-bandwidth line beside the roofline. */

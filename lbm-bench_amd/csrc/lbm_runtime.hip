@@ -85,6 +85,8 @@ struct LbmDevice {
   double *dpart, *dout;
   int npart;
   int remap, nt, wavesPerBlock, dither, rowSweep, band, wgPerCu, wallNt, closeFused;
+  int placementProbe, placementCandidates;
+  double placementGbs;
   // FP16/BF16 rounding seeds.  Every step is seeded with its absolute
   // number, steps, however the steps are grouped into graph replays and
   // eager launches, so the rounding does not depend on LBM_GRAPH_STEPS.
@@ -746,6 +748,75 @@ static void checkGeometry(const LatticeDescType *ld)
   }
 }
 
+// GB/s of the step's stream pattern (19 planes loaded and stored back in
+// place, one element per lane; the pattern kernel of lbmPatternBandwidth)
+// over a candidate PDF array, whose contents it overwrites
+static double probePlacement(LbmDevice *D, void *pdf)
+{
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  const int eb = (int)D->esize;
+  launchPattern(Q, eb, pdf, pdf, D->plane, true, D->stream);
+  HIP_OK(hipEventRecord(e0, D->stream));
+  for (int r = 0; r < 2; ++r)
+    launchPattern(Q, eb, pdf, pdf, D->plane, true, D->stream);
+  HIP_OK(hipEventRecord(e1, D->stream));
+  HIP_OK(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+  HIP_OK(hipEventDestroy(e0));
+  HIP_OK(hipEventDestroy(e1));
+  return 2.0 * 2.0 * Q * (double)D->plane * eb / ((double)ms * 1e-3) / 1e9;
+}
+
+// A PDF array.  How fast HBM serves the step's 19 streams depends on where
+// an allocation lands: over fresh 512^3 FP64 lattices the even + odd pair
+// took 15.3-15.4 ms for most and 14.0 ms for about one in four, and the
+// same pattern over the array itself (probePlacement) tells them apart
+// (5.5 against 5.9-6.0 TB/s).  For SoA arrays of at least 4 GiB, up to
+// LBM_PLACEMENT_PROBE candidates (default 6) are allocated side by side
+// while they fit in half the free device memory, each probed (three
+// launches of the pattern, about 20 ms at 512^3 FP64), and the fastest
+// kept.  (Stopping at the first candidate 4 % above the slowest kept a
+// 5.88 TB/s one whose steps ran between the two groups, so every
+// candidate is probed.)  The others are freed before the lattice is
+// initialised.
+static void *allocPlaced(LbmDevice *D, size_t bytes)
+{
+  void *best = nullptr;
+  HIP_OK(hipMalloc(&best, bytes));
+  if (D->layout != LAYOUT_SOA || D->placementProbe < 2 || bytes < (4ull << 30) ||
+      D->plane * (uint64_t)Q * D->esize != bytes)
+    return best;
+  double bestRate = probePlacement(D, best);
+  std::vector<void *> others;
+  for (int k = 1; k < D->placementProbe; ++k) {
+    size_t freeB = 0, totalB = 0;
+    HIP_OK(hipMemGetInfo(&freeB, &totalB));
+    if (bytes > freeB / 2)
+      break;
+    void *c = nullptr;
+    if (hipMalloc(&c, bytes) != hipSuccess) {
+      (void)hipGetLastError(); // no room for another candidate: keep the best so far
+      break;
+    }
+    const double r = probePlacement(D, c);
+    if (r > bestRate) {
+      others.push_back(best);
+      best     = c;
+      bestRate = r;
+    } else {
+      others.push_back(c);
+    }
+  }
+  for (void *p : others)
+    HIP_OK(hipFree(p));
+  D->placementGbs = bestRate;
+  D->placementCandidates = 1 + (int)others.size();
+  return best;
+}
+
 static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataType *cdIn,
     int prop, int layout, int tiled, const char *name,
     void (*kernel)(LatticeDescType *, KernelDataType *, CaseDataType *))
@@ -829,6 +900,9 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
   // the last step of a run adds its own body force at the store
   // (LBM_CLOSE_FUSED 0: a separate force pass after it, as before)
   D->closeFused    = envInt("LBM_CLOSE_FUSED", 1);
+  D->placementProbe      = envInt("LBM_PLACEMENT_PROBE", 6);
+  D->placementCandidates = 1;
+  D->placementGbs        = 0.0;
   // push / AA odd with nZ % 64 == 0: 1 the row sweep (one wavefront walks
   // a row), 3 the row group (one workgroup per row, nZ <= 512; else the
   // sweep), 0 the per-block kernel.  Measured at 512^3 spheres (aa-soa odd
@@ -872,7 +946,7 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
     }
   }
   for (int i = 0; i < D->narrays; ++i) {
-    HIP_OK(hipMalloc(&D->d[i], bytes));
+    D->d[i] = allocPlaced(D, bytes);
     HIP_OK(hipMemsetAsync(D->d[i], 0, bytes, D->stream));
   }
   // push-aos runs as two passes through SoA planes of post-collision
@@ -1425,6 +1499,13 @@ double lbmRunSteps(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cd, in
   kd->Kernel(ld, kd, cd);
   cd->MaxIterations = keep;
   return kd->Duration;
+}
+
+void lbmPlacement(KernelDataType *kd, int *candidates, double *gbs)
+{
+  const LbmDevice *D = kd->Dev;
+  *candidates        = D->placementCandidates;
+  *gbs               = D->placementGbs;
 }
 
 size_t lbmDeviceBytes(KernelDataType *kd)

@@ -109,6 +109,8 @@ class Library:
         L.lbmRunSteps.restype = c_double
         L.lbmDeviceBytes.argtypes = [KDP]
         L.lbmDeviceBytes.restype = ctypes.c_size_t
+        L.lbmPlacement.argtypes = [KDP, ctypes.POINTER(c_int), ctypes.POINTER(c_double)]
+        L.lbmPlacement.restype = None
         L.lbmCopyBandwidth.argtypes = [ctypes.c_size_t, c_int]
         L.lbmCopyBandwidth.restype = c_double
         L.lbmCopyBandwidthShape.argtypes = [ctypes.c_size_t, c_int, c_int, c_int, c_int]
@@ -381,6 +383,12 @@ class Lattice:
 
     def device_bytes(self):
         return self.lib.lib.lbmDeviceBytes(self.kd)
+
+    def placement(self):
+        """(candidate allocations probed, GB/s of the one kept; 1, 0.0 without a probe)"""
+        n, g = c_int(0), c_double(0.0)
+        self.lib.lib.lbmPlacement(self.kd, ctypes.byref(n), ctypes.byref(g))
+        return n.value, g.value
 
     def close(self):
         if self.kd:

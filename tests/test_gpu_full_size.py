@@ -158,3 +158,27 @@ def test_verify_reads_one_plane(gpu_lib):
         norm = L.verify()
         assert not L.kd.contents.PdfsActive
     assert np.isfinite(norm) and norm > 0
+
+
+def test_placement_probe_keeps_results(gpu_lib, monkeypatch):
+    """The 20 GB PDF array is chosen among probed candidate allocations
+    (LBM_PLACEMENT_PROBE): the probe reports what it did, and the run's
+    momentum and density fields equal those of a lattice allocated without
+    the probe exactly"""
+    with _lattice(gpu_lib) as A:
+        n, gbs = A.placement()
+        print(f"placement: {n} candidate(s), kept one at {gbs:.0f} GB/s", flush=True)
+        assert n >= 1 and gbs > 0.0
+        with monkeypatch.context() as m:
+            m.setenv("LBM_PLACEMENT_PROBE", "0")
+            B = _lattice(gpu_lib)
+        try:
+            assert B.placement() == (1, 0.0)
+            for L in (A, B):
+                L.init("random", 0, 1.0, 1e-3)
+                L.run(4)
+            cmp = A.compare_momentum(B)
+            assert cmp[0] == 0.0 and cmp[2] == 0.0 and cmp[3] == 0.0
+            assert np.array_equal(A.mass_momentum(), B.mass_momentum())
+        finally:
+            B.close()
