@@ -1040,8 +1040,9 @@ __global__ __launch_bounds__(256) void kBuildMask(const int *__restrict__ lat,
 }
 
 // bit 63 of hi: a cell of the 3x3 rows around the window (z halo included)
-// is solid, so the wavefront of this window needs bounce-back.  Reads the
-// windows from src and writes them with the flag to dst.
+// is solid, so the wavefront of this window needs bounce-back; bit
+// 2 + 3 (ox + 1) + (oy + 1): the same for the one row (x + ox, y + oy).
+// Reads the windows from src and writes them with the flags to dst.
 __global__ __launch_bounds__(256) void kMaskNeighbourhood(const MaskWin *__restrict__ src,
     MaskWin *__restrict__ dst, int nx, int ny, int segs)
 {
@@ -1052,15 +1053,16 @@ __global__ __launch_bounds__(256) void kMaskNeighbourhood(const MaskWin *__restr
   const int x   = (int)(t % nx);
   const int y   = (int)((t / nx) % ny);
   const int seg = (int)(t / ((uint64_t)nx * ny));
-  uint64_t any  = 0;
+  uint64_t rows = 0;
   for (int ox = -1; ox <= 1; ++ox)
     for (int oy = -1; oy <= 1; ++oy) {
       const int xn = wrapAdd(x, ox, nx), yn = wrapAdd(y, oy, ny);
       const MaskWin w = src[((uint64_t)xn * ny + yn) * segs + seg];
-      any |= w.lo | (w.hi & 3ull);
+      if ((w.lo | (w.hi & 3ull)) != 0)
+        rows |= 1ull << (2 + 3 * (ox + 1) + (oy + 1));
     }
   MaskWin w = src[((uint64_t)x * ny + y) * segs + seg];
-  w.hi |= (any != 0 ? 1ull : 0ull) << 63;
+  w.hi |= rows | ((rows != 0 ? 1ull : 0ull) << 63);
   dst[((uint64_t)x * ny + y) * segs + seg] = w;
 }
 

@@ -10,7 +10,9 @@
 //          w[k] = solid(x, y, (64*seg - 1 + k) mod nZ), k = 0..65,
 //          stored as lo = w[1..64], hi bit0 = w[0], hi bit1 = w[65];
 //          hi bit 63 flags a solid cell among the windows of the 3x3
-//          rows around (x, y) (bounce-back needed by this wavefront).
+//          rows around (x, y) (bounce-back needed by this wavefront),
+//          hi bit 2 + 3 (ox + 1) + (oy + 1) one in the window of row
+//          (x + ox, y + oy).
 //          Lane l of the wavefront that owns z = 64*seg + l reads its
 //          z-1, z, z+1 bits as w[l], w[l+1], w[l+2].
 // Periodicity is index wrap-around on every axis; a non-periodic axis has
@@ -236,7 +238,8 @@ struct Store<__hip_bfloat16> {
 // ---- solid mask -----------------------------------------------------------
 // lo: bit l = cell z0 + l of the row segment; hi: bit 0 = z0 - 1, bit 1 =
 // z0 + 64 (both wrapped), bit 63 = some cell of the 3x3 rows around the
-// segment (z halo included) is solid
+// segment (z halo included) is solid, bits 2 .. 10 = the same per row of
+// the 3x3 (bit 2 + 3 (ox + 1) + (oy + 1) for row (x + ox, y + oy))
 struct alignas(16) MaskWin {
   uint64_t lo, hi;
 };

@@ -924,6 +924,317 @@ __global__ __launch_bounds__(512) void kRowGroup(S *a, S *b, const MaskWin *__re
     scatter(std::true_type{});
 }
 
+// ---- vector row group: the AA odd step of 4- and 2-byte elements ----------------
+// kRowGroup's shape for elements narrower than 8 bytes.  Each lane holds
+// CPL = 8 / sizeof(S) consecutive cells of the row as one 8-byte word
+// (cell z0 + j in bits [j * 8 sizeof(S), ...) of the word), so a wavefront
+// covers 64 * CPL cells and every load and store instruction moves 512
+// line-aligned bytes, as the FP64 row group does; a 2-byte element per lane
+// moves 128 bytes, and the wavefronts of that shape (row sweep, per-block
+// kernel) left the FP16 odd step at 1.25-1.3x the even step's time.  The
+// z shift by one cell is a funnel shift of the word with the neighbouring
+// lane's word (one DPP lane shift of 32 bits), the neighbouring wavefront's
+// end word coming through LDS as in kRowGroup.  Near a solid (a wavefront
+// whose 3x3 row neighbourhood holds one) each lane reads the solid bits of
+// its cells z0 - 1 .. z0 + CPL from the mask windows and stores per cell
+// where only some of its cells may be written: a streamed store is skipped
+// for a solid producer (its fluid target fills the slot by bounce-back) and
+// a bounce-back store is made for a solid target, as in the other kernels.
+// The loads complete before the first barrier (their values reach LDS or
+// the collision before the second) and every store follows the second, so
+// the slots that move between wavefronts are read before they are written.
+template <int NT, typename S, int WB = 8, bool CL = false>
+__global__ __launch_bounds__(512) void kRowGroupV(S *a, const MaskWin *__restrict__ mask,
+    Params<S> p, int remap)
+{
+  static_assert(sizeof(S) == 2 || sizeof(S) == 4, "vector row group: 4- and 2-byte elements");
+  static_assert(!CL || closable<S>(), "closing: storage without stochastic rounding");
+  using R  = typename Store<S>::R;
+  using St = Store<S>;
+  using E  = std::conditional_t<sizeof(S) == 2, unsigned short, unsigned int>;
+  using Wd = std::conditional_t<WB == 8, uint64_t, uint32_t>; // the lane's word
+  static_assert(WB == 8 || (WB == 4 && sizeof(S) == 2), "words of 8 bytes, or 4 for 2-byte elements");
+  constexpr int CPL   = WB / (int)sizeof(S);
+  constexpr int BITS  = 8 * (int)sizeof(S);
+  constexpr int SH    = 32 - BITS; // funnel shift of a one-cell move toward higher z
+  constexpr uint32_t FULL = (1u << CPL) - 1u;
+  constexpr bool NTS = NT & 1, NTL = NT & 2;
+  constexpr int E0 = T; // the ten directions with c_z != 0 are T .. BE
+  __shared__ uint32_t xg[8][10], xs[8][10];
+
+  const int row  = (int)(remap ? xcdLogicalBlock(blockIdx.x, gridDim.x) : blockIdx.x);
+  const int w    = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int nw   = (int)(blockDim.x >> 6);
+  const int lane = (int)(threadIdx.x & 63u);
+  const int nz = p.nz, K = p.segs;
+  const int wl = w == 0 ? nw - 1 : w - 1, wr = w == nw - 1 ? 0 : w + 1;
+  const int x = row / p.ny, y = row - x * p.ny;
+  const int xs_[3] = { wrapDec(x, p.nx), x, wrapInc(x, p.nx) };
+  const int ys_[3] = { wrapDec(y, p.ny), y, wrapInc(y, p.ny) };
+  auto rowOf = [&](int ox, int oy) -> uint64_t {
+    return (uint64_t)xs_[ox + 1] * (uint64_t)p.ny + (uint64_t)ys_[oy + 1];
+  };
+  // slot offsets from strides the compiler cannot see through (as in kRowGroup)
+  auto sl = [&](uint64_t c, int d) -> uint64_t {
+    return c + (uint64_t)(uint32_t)d * (uint64_t)opaqueS((uint32_t)p.plane);
+  };
+  auto rowStart = [&](int ox, int oy) -> uint64_t {
+    const uint32_t r = (uint32_t)xs_[ox + 1] * (uint32_t)p.ny + (uint32_t)ys_[oy + 1];
+    return (uint64_t)(r * opaqueS((uint32_t)nz));
+  };
+  const int z0 = (w * 64 + lane) * CPL;
+  // the lane's word of slot d in row (ox, oy): cells z0 .. z0 + CPL - 1
+  auto wordPtr = [&](int ox, int oy, int d) -> Wd * {
+    return (Wd *)(a + sl(rowStart(ox, oy) + (uint64_t)z0, d));
+  };
+  auto ld = [&](const Wd *ptr, auto nt) -> Wd {
+    if constexpr (decltype(nt)::value)
+      return __builtin_nontemporal_load(ptr);
+    else
+      return *ptr;
+  };
+  auto st = [&](Wd *ptr, Wd v) {
+    if constexpr (NTS)
+      __builtin_nontemporal_store(v, ptr);
+    else
+      *ptr = v;
+  };
+  // cells j with bit j of m set get their value from v (per cell where not all)
+  auto stMasked = [&](Wd *ptr, Wd v, uint32_t m) {
+    if (m == FULL) {
+      st(ptr, v);
+    } else if (m != 0u) {
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) {
+        if ((m >> j) & 1u) {
+          const E e = (E)(v >> (BITS * j));
+          if constexpr (NTS)
+            __builtin_nontemporal_store(e, (E *)ptr + j);
+          else
+            ((E *)ptr)[j] = e;
+        }
+      }
+    }
+  };
+  // one-cell moves of a word: toward higher z (cell z takes z - 1; `prev`
+  // is the top 32 bits of the word below) and toward lower z (cell z takes
+  // z + 1; `next` is the bottom 32 bits of the word above)
+  auto funnel = [](uint32_t hi, uint32_t lo, int s) -> uint32_t {
+    return (uint32_t)((((uint64_t)hi << 32) | lo) >> s);
+  };
+  auto moveUp = [&](Wd v, uint32_t prev) -> Wd {
+    if constexpr (WB == 8) {
+      const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+      return ((uint64_t)funnel(hi, lo, SH) << 32) | funnel(lo, prev, SH);
+    } else {
+      return funnel(v, prev, SH);
+    }
+  };
+  auto moveDown = [&](Wd v, uint32_t next) -> Wd {
+    if constexpr (WB == 8) {
+      const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+      return ((uint64_t)funnel(next, hi, BITS) << 32) | funnel(hi, lo, BITS);
+    } else {
+      return funnel(next, v, BITS);
+    }
+  };
+  auto top32 = [](Wd v) -> uint32_t { return (uint32_t)(v >> (8 * WB - 32)); };
+  // top 32 bits of lane l - 1's word (lane 0: fill) / bottom 32 bits of
+  // lane l + 1's word (lane 63: fill)
+  auto topOfBelow = [&](Wd v, uint32_t fill) -> uint32_t {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)top32(v), 0x138, 0xf, 0xf, false);
+  };
+  auto bottomOfAbove = [](Wd v, uint32_t fill) -> uint32_t {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)(uint32_t)v, 0x130, 0xf, 0xf, false);
+  };
+
+  // The wavefront's CPL mask windows of its own row (scalar).  clean: no
+  // solid cell in the 3x3 rows around its cells (z halo included).  Near a
+  // solid, the per-row flags of the windows say which 64-cell groups of
+  // lanes may need a bounce-back value from a row: those lanes load their
+  // own slot (one line per group), in the same round trip as the streamed
+  // loads, and the per-cell solid bits come from per-lane window loads
+  // issued with them.
+  constexpr int LPW = 64 / CPL; // lanes per mask window
+  MaskWin ow[CPL];
+  bool clean = true;
+#pragma unroll
+  for (int i = 0; i < CPL; ++i) {
+    ow[i] = mask[rowOf(0, 0) * K + w * CPL + i];
+    clean = clean && !nearSolid(ow[i]);
+  }
+  // lanes whose window flags a solid in row (ox, oy)
+  auto groupLanes = [&](int ox, int oy) -> uint64_t {
+    const int r = 2 + 3 * (ox + 1) + (oy + 1);
+    uint64_t m  = 0;
+#pragma unroll
+    for (int i = 0; i < CPL; ++i)
+      m |= ((ow[i].hi >> r) & 1ull) ? (~0ull >> (64 - LPW)) << (LPW * i) : 0ull;
+    return m;
+  };
+  // solid bits of cells z0 - 1 .. z0 + CPL of the nine rows (ox, oy): bit
+  // i is cell z0 - 1 + i (wrapped in z through the window's halo bits)
+  const int seg = z0 >> 6, b = z0 & 63;
+  MaskWin mw[9];
+  uint32_t fld9[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+    fld9[i] = 0u;
+  auto fields = [&]() {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const uint32_t own   = (uint32_t)(mw[i].lo >> b) & FULL;
+      const uint32_t below = b == 0 ? (uint32_t)(mw[i].hi & 1ull) : (uint32_t)(mw[i].lo >> (b - 1)) & 1u;
+      const uint32_t above =
+          b + CPL == 64 ? (uint32_t)(mw[i].hi >> 1) & 1u : (uint32_t)(mw[i].lo >> (b + CPL)) & 1u;
+      fld9[i] = below | (own << 1) | (above << (CPL + 1));
+    }
+  };
+  auto field = [&](int ox, int oy) -> uint32_t { return fld9[(ox + 1) * 3 + oy + 1]; };
+  // the CPL bits of cells z0 + j + oz, j = 0 .. CPL - 1
+  auto cellsAt = [&](uint32_t fld, int oz) -> uint32_t { return (fld >> (1 + oz)) & FULL; };
+
+  // gather from x - c_d: slot inv d of the neighbour row, moved by -c_z;
+  // bounce-back cells take their own slot d
+  Wd v[Q], own[Q];
+  auto gather = [&](auto walls) {
+    constexpr bool W = decltype(walls)::value;
+    if constexpr (W) {
+#pragma unroll
+      for (int i = 0; i < 9; ++i)
+        mw[i] = mask[rowOf(i / 3 - 1, i % 3 - 1) * K + seg];
+    }
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+      v[d]   = ld(wordPtr(-cx(d), -cy(d), inv(d)), std::integral_constant<bool, NTL>{});
+      own[d] = 0;
+      if (W && d != C && __builtin_amdgcn_inverse_ballot_w64(groupLanes(-cx(d), -cy(d))))
+        own[d] = ld(wordPtr(0, 0, d), std::integral_constant<bool, NTL>{});
+    }
+  };
+  if (clean)
+    gather(std::false_type{});
+  else
+    gather(std::true_type{});
+#pragma unroll
+  for (int d = E0; d < E0 + 10; ++d) {
+    if (cz(d) > 0 && lane == 63)
+      xg[w][d - E0] = top32(v[d]);
+    if (cz(d) < 0 && lane == 0)
+      xg[w][d - E0] = (uint32_t)v[d];
+  }
+  __syncthreads();
+  if (!clean)
+    fields();
+  auto take = [&](auto walls) {
+    constexpr bool W = decltype(walls)::value;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+      if (cz(d) > 0)
+        v[d] = moveUp(v[d], topOfBelow(v[d], xg[wl][d - E0]));
+      else if (cz(d) < 0)
+        v[d] = moveDown(v[d], bottomOfAbove(v[d], xg[wr][d - E0]));
+      if (W && d != C) {
+        const uint32_t m = cellsAt(field(-cx(d), -cy(d)), -cz(d));
+        Wd sel           = 0;
+#pragma unroll
+        for (int j = 0; j < CPL; ++j)
+          sel |= ((m >> j) & 1u) ? (Wd)(((1ull << BITS) - 1ull) << (BITS * j)) : (Wd)0;
+        v[d] = (own[d] & sel) | (v[d] & ~sel);
+      }
+    }
+  };
+  if (clean)
+    take(std::false_type{});
+  else
+    take(std::true_type{});
+
+  // collide the lane's cells one after the other; out: streamed values,
+  // bb: bounce-back values (they differ in a closing step only)
+  Wd out[Q], bb[Q];
+#pragma unroll
+  for (int d = 0; d < Q; ++d) {
+    out[d] = 0;
+    bb[d]  = 0;
+  }
+  const uint32_t seqv = stepSeq(p);
+  const uint64_t rc   = rowStart(0, 0);
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) {
+    R f[Q];
+#pragma unroll
+    for (int d = 0; d < Q; ++d)
+      f[d] = St::raw(__builtin_bit_cast(S, (E)(v[d] >> (BITS * j))));
+    if (p.addForce)
+      addForce(f, p.f1, p.f2);
+    collide(f, p.trt);
+#pragma unroll
+    for (int d = 0; d < Q; ++d)
+      asm("" : "+v"(f[d]));
+    const Dither dz = ditherOf(rc + (uint64_t)(z0 + j), seqv, p.dither);
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+      R fv = f[d], fb = f[d];
+      if constexpr (CL) {
+        if (cx(d) != 0) {
+          fv += closeTerm(d, p.cf1, p.cf2);
+          fb -= closeTerm(d, p.cf1, p.cf2);
+        }
+      }
+      const E ev = __builtin_bit_cast(E, St::rawStep(fv, ditherDir(dz, d)));
+      out[d] |= (Wd)ev << (BITS * j);
+      if constexpr (CL) {
+        const E eb = __builtin_bit_cast(E, St::rawStep(fb, ditherDir(dz, d)));
+        bb[d] |= (Wd)eb << (BITS * j);
+      }
+    }
+  }
+  if constexpr (!CL) {
+#pragma unroll
+    for (int d = 0; d < Q; ++d)
+      bb[d] = out[d];
+  }
+#pragma unroll
+  for (int d = E0; d < E0 + 10; ++d) {
+    if (cz(d) > 0 && lane == 63)
+      xs[w][d - E0] = top32(out[d]);
+    if (cz(d) < 0 && lane == 0)
+      xs[w][d - E0] = (uint32_t)out[d];
+  }
+  __syncthreads();
+
+  // stores to x + c_d, the z-shifted ones moved by +c_z: the lane's word of
+  // the target row takes the values of cells z0 + j - c_z
+  auto scatter = [&](auto walls) {
+    constexpr bool W = decltype(walls)::value;
+    const uint32_t fl = W ? FULL & ~cellsAt(field(0, 0), 0) : FULL; // fluid cells of the lane
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+      if (d == C) {
+        stMasked(wordPtr(0, 0, C), out[C], fl);
+        continue;
+      }
+      if (W) {
+        // bounce-back: fluid cells whose target x + c_d is solid, own slot inv d
+        const uint32_t m = fl & cellsAt(field(cx(d), cy(d)), cz(d));
+        stMasked(wordPtr(0, 0, inv(d)), bb[d], m);
+      }
+      Wd sv = out[d];
+      if (cz(d) > 0)
+        sv = moveUp(sv, topOfBelow(sv, xs[wl][d - E0]));
+      else if (cz(d) < 0)
+        sv = moveDown(sv, bottomOfAbove(sv, xs[wr][d - E0]));
+      // producers of the word's cells: z0 + j - c_z, skipped where solid
+      const uint32_t m = W ? FULL & ~cellsAt(field(0, 0), -cz(d)) : FULL;
+      stMasked(wordPtr(cx(d), cy(d), d), sv, m);
+    }
+  };
+  if (clean)
+    scatter(std::false_type{});
+  else
+    scatter(std::true_type{});
+}
+
 template <int V, int NT, typename S>
 static void launchRowGroup(const StepLaunch &L, hipStream_t s)
 {
@@ -960,6 +1271,23 @@ static void launchRowGroup(const StepLaunch &L, hipStream_t s)
 #undef LAUNCH_RG
 }
 
+template <int NT, typename S, int WB>
+static void launchRowGroupV(const StepLaunch &L, hipStream_t s)
+{
+  Params<S> p           = makeParams<S>(L);
+  const unsigned blocks = (unsigned)((uint64_t)L.g.nx * L.g.ny);
+  const unsigned lanes  = (unsigned)(L.g.nz / (WB / (int)sizeof(S))); // 64 per wavefront
+  if constexpr (closable<S>()) {
+    if (L.close) {
+      hipLaunchKernelGGL((kRowGroupV<1, S, WB, true>), dim3(blocks), dim3(lanes), 0, s,
+          (S *)L.src, L.g.mask, p, L.remap);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((kRowGroupV<NT, S, WB>), dim3(blocks), dim3(lanes), 0, s, (S *)L.src,
+      L.g.mask, p, L.remap);
+}
+
 template <int V, int NT, typename S>
 static void launchRowSweep(const StepLaunch &L, hipStream_t s)
 {
@@ -985,6 +1313,19 @@ static void launchT(const StepLaunch &L, hipStream_t s)
   } else {
     if constexpr (V == V_PUSH || V == V_AA_ODD) {
       if (L.rowSweep && L.g.nz % 64 == 0 && L.g.plane < (1ull << 32)) {
+        if constexpr (V == V_AA_ODD && sizeof(S) < 8) {
+          // whole words per lane (8 bytes; 4 bytes for 2-byte elements with
+          // LBM_ROW_SWEEP=4), at most 8 wavefronts per row
+          constexpr int C8 = 8 / (int)sizeof(S), C4 = 4 / (int)sizeof(S);
+          if (L.rowSweep == 3 && !L.tiled && L.g.nz % (64 * C8) == 0 && L.g.nz <= 512 * C8 &&
+              L.g.plane % C8 == 0)
+            return launchRowGroupV<NT, S, 8>(L, s);
+          if constexpr (sizeof(S) == 2) {
+            if (L.rowSweep == 4 && !L.tiled && L.g.nz % (64 * C4) == 0 && L.g.nz <= 512 * C4 &&
+                L.g.plane % C4 == 0)
+              return launchRowGroupV<NT, S, 4>(L, s);
+          }
+        }
         if (L.rowSweep == 3 && L.g.segs <= 8)
           return launchRowGroup<V, NT, S>(L, s);
         return launchRowSweep<V, NT, S>(L, s);
