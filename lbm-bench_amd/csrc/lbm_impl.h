@@ -132,15 +132,11 @@ __device__ __forceinline__ void storeRow(S *o, uint64_t plane, uint64_t rowCell,
 // stores at x is the logical f_inv d of x.  The last step of a run can
 // therefore add its own body force at the store, +t_d to a streamed value
 // and -t_d to a bounce-back one, as the force pass after the step would
-// (src/lbm.c:902-922): the force pass loads the stored value and adds the
-// term with one rounding, which is the same arithmetic when the stored
-// value is the collided one, i.e. storage without stochastic rounding.
-template <typename S>
-constexpr bool closable()
-{
-  return std::is_same<S, double>::value || std::is_same<S, float>::value ||
-         std::is_same<S, f32dev>::value;
-}
+// (src/lbm.c:902-922).  The force pass loads the stored value and adds the
+// term with one rounding: for storage without stochastic rounding (FP64,
+// FP32, FP32 deviation) that is the same arithmetic, bit for bit; FP16 and
+// BF16 storage round f*_d + t_d once, stochastically, where the force pass
+// rounds f*_d and then the sum.
 // t_d for a direction with c_x != 0
 template <typename R>
 __device__ __forceinline__ R closeTerm(int d, R cf1, R cf2)

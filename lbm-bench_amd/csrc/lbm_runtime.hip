@@ -628,13 +628,11 @@ static void runKernel(LatticeDescType *ld, KernelDataType *kd, CaseDataType *cdI
     D->tkind.assign(1, -1);
   }
   // The last step adds its own force at the store where it can (push and
-  // AA odd on SoA storage without stochastic rounding, no pressure
-  // boundaries after it): the closing force pass then drops out.  That
-  // step is launched directly, after the graph replays.
+  // AA odd on SoA storage, no pressure boundaries after it): the closing
+  // force pass then drops out.  That step is launched directly, after the
+  // graph replays.
   const bool fuse = bf && !D->zhOn && D->layout == LAYOUT_SOA &&
-                    (D->prop == PROP_AA || D->prop == PROP_PUSH) &&
-                    (D->storage == ST_F64 || D->storage == ST_F32 || D->storage == ST_F32DEV) &&
-                    D->closeFused;
+                    (D->prop == PROP_AA || D->prop == PROP_PUSH) && D->closeFused;
   if (n > 0) {
     int left = n;
     if (D->prop == PROP_AA) {

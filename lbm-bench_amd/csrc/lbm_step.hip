@@ -46,7 +46,7 @@ template <int V, int NT, typename S, bool CL = false>
 __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const MaskWin *__restrict__ mask,
     Params<S> p, int remap, int tiled)
 {
-  static_assert(!CL || ((V == V_PUSH || V == V_AA_ODD) && closable<S>()), "closing: push / AA odd");
+  static_assert(!CL || V == V_PUSH || V == V_AA_ODD, "closing: push / AA odd");
   using R = typename Store<S>::R;
   using St = Store<S>;
   constexpr bool NTS = NT & 1, NTL = NT & 2;
@@ -492,7 +492,6 @@ template <int V, int NT, typename S, bool CL = false>
 __global__ __launch_bounds__(256, LBM_MIN_WAVES) void kRowSweep(S *a, S *b,
     const MaskWin *__restrict__ mask, Params<S> p, int remap, int tiled)
 {
-  static_assert(!CL || closable<S>(), "closing: storage without stochastic rounding");
   using R  = typename Store<S>::R;
   using St = Store<S>;
   constexpr bool NTS = NT & 1, NTL = NT & 2;
@@ -744,7 +743,6 @@ template <int V, int NT, typename S, int WNT = (NT | ((NT & 2) << 1)), bool CL =
 __global__ __launch_bounds__(512) void kRowGroup(S *a, S *b, const MaskWin *__restrict__ mask,
     Params<S> p, int tiled, int remap)
 {
-  static_assert(!CL || closable<S>(), "closing: storage without stochastic rounding");
   using R  = typename Store<S>::R;
   using St = Store<S>;
   constexpr bool NTS = NT & 1, NTL = NT & 2;
@@ -948,7 +946,6 @@ __global__ __launch_bounds__(512) void kRowGroupV(S *a, const MaskWin *__restric
     Params<S> p, int remap)
 {
   static_assert(sizeof(S) == 2 || sizeof(S) == 4, "vector row group: 4- and 2-byte elements");
-  static_assert(!CL || closable<S>(), "closing: storage without stochastic rounding");
   using R  = typename Store<S>::R;
   using St = Store<S>;
   using E  = std::conditional_t<sizeof(S) == 2, unsigned short, unsigned int>;
@@ -1243,16 +1240,14 @@ static void launchRowGroup(const StepLaunch &L, hipStream_t s)
 #define LAUNCH_RG(W)                                                                         \
   hipLaunchKernelGGL((kRowGroup<V, NT, S, W>), dim3(blocks), dim3(64 * L.g.segs), 0, s,     \
       (S *)L.src, (S *)L.dst, L.g.mask, p, L.tiled, L.remap)
-  if constexpr (closable<S>()) {
-    if (L.close) {
-      // the closing step: one compilation per variant and storage, with the
-      // default non-temporal bits (they do not change results)
-      constexpr int NTC = sizeof(S) == 8 ? 3 : 1;
-      constexpr int WC  = V == V_AA_ODD && sizeof(S) == 8 ? 1 : (NTC | ((NTC & 2) << 1));
-      hipLaunchKernelGGL((kRowGroup<V, NTC, S, WC, true>), dim3(blocks), dim3(64 * L.g.segs), 0,
-          s, (S *)L.src, (S *)L.dst, L.g.mask, p, L.tiled, L.remap);
-      return;
-    }
+  if (L.close) {
+    // the closing step: one compilation per variant and storage, with the
+    // default non-temporal bits (they do not change results)
+    constexpr int NTC = sizeof(S) == 8 ? 3 : 1;
+    constexpr int WC  = V == V_AA_ODD && sizeof(S) == 8 ? 1 : (NTC | ((NTC & 2) << 1));
+    hipLaunchKernelGGL((kRowGroup<V, NTC, S, WC, true>), dim3(blocks), dim3(64 * L.g.segs), 0,
+        s, (S *)L.src, (S *)L.dst, L.g.mask, p, L.tiled, L.remap);
+    return;
   }
   if constexpr (V == V_AA_ODD && sizeof(S) == 8 && NT == 3) {
     switch (L.wallNt) {
@@ -1277,12 +1272,10 @@ static void launchRowGroupV(const StepLaunch &L, hipStream_t s)
   Params<S> p           = makeParams<S>(L);
   const unsigned blocks = (unsigned)((uint64_t)L.g.nx * L.g.ny);
   const unsigned lanes  = (unsigned)(L.g.nz / (WB / (int)sizeof(S))); // 64 per wavefront
-  if constexpr (closable<S>()) {
-    if (L.close) {
-      hipLaunchKernelGGL((kRowGroupV<1, S, WB, true>), dim3(blocks), dim3(lanes), 0, s,
-          (S *)L.src, L.g.mask, p, L.remap);
-      return;
-    }
+  if (L.close) {
+    hipLaunchKernelGGL((kRowGroupV<1, S, WB, true>), dim3(blocks), dim3(lanes), 0, s,
+        (S *)L.src, L.g.mask, p, L.remap);
+    return;
   }
   hipLaunchKernelGGL((kRowGroupV<NT, S, WB>), dim3(blocks), dim3(lanes), 0, s, (S *)L.src,
       L.g.mask, p, L.remap);
@@ -1293,13 +1286,11 @@ static void launchRowSweep(const StepLaunch &L, hipStream_t s)
 {
   Params<S> p           = makeParams<S>(L);
   const unsigned blocks = L.tiled ? tileBlocks(L.g, 1) : (unsigned)(((uint64_t)L.g.nx * L.g.ny + 3) / 4);
-  if constexpr (closable<S>()) {
-    if (L.close) {
-      constexpr int NTC = sizeof(S) == 8 ? 3 : 1;
-      hipLaunchKernelGGL((kRowSweep<V, NTC, S, true>), dim3(blocks), dim3(256), 0, s, (S *)L.src,
-          (S *)L.dst, L.g.mask, p, L.remap, L.tiled);
-      return;
-    }
+  if (L.close) {
+    constexpr int NTC = sizeof(S) == 8 ? 3 : 1;
+    hipLaunchKernelGGL((kRowSweep<V, NTC, S, true>), dim3(blocks), dim3(256), 0, s, (S *)L.src,
+        (S *)L.dst, L.g.mask, p, L.remap, L.tiled);
+    return;
   }
   hipLaunchKernelGGL((kRowSweep<V, NT, S>), dim3(blocks), dim3(256), 0, s, (S *)L.src,
       (S *)L.dst, L.g.mask, p, L.remap, L.tiled);
@@ -1334,7 +1325,7 @@ static void launchT(const StepLaunch &L, hipStream_t s)
     Params<S> p       = makeParams<S>(L);
     const int wpb     = L.tiled ? 4 : (L.wavesPerBlock == 8 ? 8 : 4);
     const unsigned blocks = L.tiled ? tileBlocks(L.g, L.g.segs) : rowBlocks(L.g, wpb);
-    if constexpr ((V == V_PUSH || V == V_AA_ODD) && closable<S>()) {
+    if constexpr (V == V_PUSH || V == V_AA_ODD) {
       if (L.close) {
         constexpr int NTC = sizeof(S) == 8 ? 3 : 1;
         hipLaunchKernelGGL((kStepRow<V, NTC, S, true>), dim3(blocks), dim3(64 * wpb), 0, s,

@@ -15,6 +15,7 @@ the collision differently: there the bound is 1e-14 relative (FP64) or a
 few FP16 quanta of the stored deviation (FP16)."""
 import numpy as np
 import pytest
+from conftest import WEIGHTS
 
 from lbmhip import Lattice
 
@@ -105,3 +106,27 @@ def test_closing_step_equals_force_pass(request, monkeypatch, precision, storage
             same = np.array_equal(fused[fluid].view(w), ref[fluid].view(w))
             assert same, (kernel, storage, precision, steps, sweep,
                           float(np.abs(fused - ref)[fluid].max()))
+
+
+@pytest.mark.parametrize("storage,u", [("fp16", 2.0 ** -11), ("bf16", 2.0 ** -8)])
+@pytest.mark.parametrize("kernel", ["aa-soa", "push-soa", "blk-push-soa"])
+def test_closing_step_rounds_once(request, monkeypatch, storage, u, kernel):
+    """With stochastically rounded storage the closing step rounds the
+    collided value plus the force once, where the force pass rounds the
+    collided value and then the sum: the fluid PDFs differ by the last
+    rounding only, after runs of one, an even and odd number of steps and
+    more than a step graph, for the row group, row sweep and per-block
+    kernels"""
+    lib = request.getfixturevalue("gpu_lib")
+    for steps in (1, 2, 7, 40):
+        for sweep in ("3", "1", "0"):
+            with monkeypatch.context() as m:
+                m.setenv("LBM_ROW_SWEEP", sweep)
+                fused, fluid = _state_fluid(lib, kernel, storage, steps)
+                m.setenv("LBM_CLOSE_FUSED", "0")
+                ref, _ = _state_fluid(lib, kernel, storage, steps)
+            assert np.isfinite(fused[fluid]).all()
+            dev = np.abs(ref - WEIGHTS)[fluid].max()
+            diff = float(np.abs(fused - ref)[fluid].max())
+            assert diff <= 4 * u * dev, (kernel, storage, steps, sweep, diff, dev)
+
