@@ -908,7 +908,7 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
   // 4 for 2-byte AA odd the row group of 4-byte words, 0 the per-block
   // kernel.  Measured at 512^3 spheres (aa-soa odd step, one process):
   // 8-byte elements row group 8.40 ms, sweep 8.80, per-block 9.30; 2-byte
-  // sweep 2.32, row group 3.75, in 8-byte words 2.50, in 4-byte words 2.44;
+  // sweep 2.32, row group 3.75, in 8-byte words 2.50, in 4-byte words 2.31;
   // 4-byte per-block 3.98, row group 4.51, sweep 4.69, in 8-byte words
   // 4.3.  Unset for 2-byte AA storage: chosen from the solids (below).
   D->rowSweep      = envInt("LBM_ROW_SWEEP", D->esize == 8 ? 3 : (D->esize == 4 ? 0 : 1));
@@ -984,30 +984,31 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
   HIP_OK(hipFree(dlat));
 
   // The AA odd step of 2-byte storage, LBM_ROW_SWEEP unset: the row group of
-  // 8-byte words (kRowGroupV, four cells per lane) runs a wavefront with no
-  // solid nearby in the time of the even step, one near a solid in about
-  // 1.6x that (it masks stores per cell); the row sweep's time depends
-  // little on the solids.  Measured at 512^3 (odd step, FP16): plain channel
-  // (0.8 % of the 256-cell wavefronts near a solid) 1.79 ms against the
-  // sweep's 2.08; 2000 spheres (69 %) 2.50 ms against 2.30.  The two lines
-  // cross near 39 %; below 35 % the word row group runs.
+  // 4-byte words (kRowGroupV, two cells per lane) runs a wavefront with no
+  // solid nearby about as fast as the even step and one near a solid more
+  // slowly (it masks stores per cell); the row sweep depends less on the
+  // solids.  Measured at 512^3 (FP16 odd step, words against sweep, by the
+  // share of 128-cell wavefronts near a solid): plain channel (0.8 %) 1.83
+  // against 2.08 ms; 2000 spheres (46 %) 2.31-2.37 against 2.27-2.35; 6000
+  // spheres (83 %) 2.67 against 2.57; 15000 spheres (99 %) 3.0 against 2.9.
+  // Below 45 % the word row group runs.
   if (envInt("LBM_ROW_SWEEP", -1) < 0 && D->esize == 2 && D->prop == PROP_AA &&
-      D->layout == LAYOUT_SOA && D->nz % 256 == 0 && D->nz <= 2048 && D->plane % 4 == 0 &&
+      D->layout == LAYOUT_SOA && D->nz % 128 == 0 && D->nz <= 1024 && D->plane % 2 == 0 &&
       D->plane < (1ull << 32)) {
-    const int waves = D->nz / 256;
+    const int waves = D->nz / 128;
     uint64_t near   = 0;
     for (size_t r = 0; r < (size_t)D->nx * D->ny; ++r)
       for (int w = 0; w < waves; ++w) {
         uint64_t any = 0;
-        for (int i = 0; i < 4; ++i)
-          any |= D->hmask[r * D->segs + (size_t)w * 4 + i].hi >> 63;
+        for (int i = 0; i < 2; ++i)
+          any |= D->hmask[r * D->segs + (size_t)w * 2 + i].hi >> 63;
         near += any;
       }
     const double frac = (double)near / ((double)D->nx * D->ny * waves);
-    if (frac < 0.35)
-      D->rowSweep = 3;
-    printf("# AA odd step: %s (%.1f %% of the 256-cell wavefronts near a solid)\n",
-        D->rowSweep == 3 ? "row group of 8-byte words" : "row sweep", 100.0 * frac);
+    if (frac < 0.45)
+      D->rowSweep = 4;
+    printf("# AA odd step: %s (%.1f %% of the 128-cell wavefronts near a solid)\n",
+        D->rowSweep == 4 ? "row group of 4-byte words" : "row sweep", 100.0 * frac);
   }
 
   if (!ensureScratch())

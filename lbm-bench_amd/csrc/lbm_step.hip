@@ -980,9 +980,16 @@ __global__ __launch_bounds__(512) void kRowGroupV(S *a, const MaskWin *__restric
     return (uint64_t)(r * opaqueS((uint32_t)nz));
   };
   const int z0 = (w * 64 + lane) * CPL;
-  // the lane's word of slot d in row (ox, oy): cells z0 .. z0 + CPL - 1
+  // the lane's word of slot d in row (ox, oy): cells z0 .. z0 + CPL - 1,
+  // as a scalar base (the row's start in the plane) plus the lane's 32-bit
+  // byte offset, so an access costs no vector address arithmetic and all
+  // of them share one offset register
+  const uint32_t zoff = (uint32_t)z0 * (uint32_t)sizeof(S);
   auto wordPtr = [&](int ox, int oy, int d) -> Wd * {
-    return (Wd *)(a + sl(rowStart(ox, oy) + (uint64_t)z0, d));
+    typedef __attribute__((address_space(1))) char gchar;
+    gchar *base = (gchar *)(a + sl(rowStart(ox, oy), d));
+    asm("" : "+s"(base));
+    return (Wd *)(base + zoff);
   };
   auto ld = [&](const Wd *ptr, auto nt) -> Wd {
     if constexpr (decltype(nt)::value)
@@ -1108,6 +1115,10 @@ __global__ __launch_bounds__(512) void kRowGroupV(S *a, const MaskWin *__restric
       if (W && d != C && __builtin_amdgcn_inverse_ballot_w64(groupLanes(-cx(d), -cy(d))))
         own[d] = ld(wordPtr(0, 0, d), std::integral_constant<bool, NTL>{});
     }
+    // the window loads went out first: waiting for them leaves the
+    // gather in flight, and the windows' registers free before the barrier
+    if constexpr (W)
+      fields();
   };
   if (clean)
     gather(std::false_type{});
@@ -1121,8 +1132,6 @@ __global__ __launch_bounds__(512) void kRowGroupV(S *a, const MaskWin *__restric
       xg[w][d - E0] = (uint32_t)v[d];
   }
   __syncthreads();
-  if (!clean)
-    fields();
   auto take = [&](auto walls) {
     constexpr bool W = decltype(walls)::value;
 #pragma unroll
@@ -1147,13 +1156,15 @@ __global__ __launch_bounds__(512) void kRowGroupV(S *a, const MaskWin *__restric
     take(std::true_type{});
 
   // collide the lane's cells one after the other; out: streamed values,
-  // bb: bounce-back values (they differ in a closing step only)
-  Wd out[Q], bb[Q];
+  // written over the gathered word cell by cell (a cell's gathered value
+  // is not used after its own collision, and a second array of words
+  // would cost the registers of a wavefront per SIMD); bb: bounce-back
+  // values (they differ in a closing step only)
+  Wd(&out)[Q] = v;
+  Wd bb[Q];
 #pragma unroll
-  for (int d = 0; d < Q; ++d) {
-    out[d] = 0;
-    bb[d]  = 0;
-  }
+  for (int d = 0; d < Q; ++d)
+    bb[d] = 0;
   const uint32_t seqv = stepSeq(p);
   const uint64_t rc   = rowStart(0, 0);
 #pragma unroll
@@ -1179,7 +1190,7 @@ __global__ __launch_bounds__(512) void kRowGroupV(S *a, const MaskWin *__restric
         }
       }
       const E ev = __builtin_bit_cast(E, St::rawStep(fv, ditherDir(dz, d)));
-      out[d] |= (Wd)ev << (BITS * j);
+      out[d] = (out[d] & ~((Wd)(E)~E(0) << (BITS * j))) | ((Wd)ev << (BITS * j));
       if constexpr (CL) {
         const E eb = __builtin_bit_cast(E, St::rawStep(fb, ditherDir(dz, d)));
         bb[d] |= (Wd)eb << (BITS * j);
