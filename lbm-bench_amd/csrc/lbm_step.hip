@@ -924,15 +924,15 @@ __global__ __launch_bounds__(512) void kRowGroup(S *a, S *b, const MaskWin *__re
 
 // ---- vector row group: the AA odd step of 4- and 2-byte elements ----------------
 // kRowGroup's shape for elements narrower than 8 bytes.  Each lane holds
-// CPL = 8 / sizeof(S) consecutive cells of the row as one 8-byte word
-// (cell z0 + j in bits [j * 8 sizeof(S), ...) of the word), so a wavefront
-// covers 64 * CPL cells and every load and store instruction moves 512
-// line-aligned bytes, as the FP64 row group does; a 2-byte element per lane
-// moves 128 bytes, and the wavefronts of that shape (row sweep, per-block
-// kernel) left the FP16 odd step at 1.25-1.3x the even step's time.  The
-// z shift by one cell is a funnel shift of the word with the neighbouring
-// lane's word (one DPP lane shift of 32 bits), the neighbouring wavefront's
-// end word coming through LDS as in kRowGroup.  Near a solid (a wavefront
+// CPL = WB / sizeof(S) consecutive cells of the row as one word of WB
+// bytes (8, or 4 for 2-byte elements; cell z0 + j in bits
+// [j * 8 sizeof(S), ...) of the word), so a wavefront covers 64 * CPL cells
+// and every load and store instruction moves 64 * WB line-aligned bytes; a
+// 2-byte element per lane moves 128 bytes, and the wavefronts of that shape
+// (row sweep, per-block kernel) do more address and shift work per cell.
+// The z shift by one cell is a funnel shift of the word with the
+// neighbouring lane's word (one DPP lane shift of 32 bits), the
+// neighbouring wavefront's end word coming through LDS as in kRowGroup.  Near a solid (a wavefront
 // whose 3x3 row neighbourhood holds one) each lane reads the solid bits of
 // its cells z0 - 1 .. z0 + CPL from the mask windows and stores per cell
 // where only some of its cells may be written: a streamed store is skipped
