@@ -162,6 +162,19 @@ __device__ __forceinline__ Dither ditherDir(Dither z, int d)
 {
   return Dither{ z.mask, z.salt ^ (((uint32_t)d + 1u) * 0x9e3779b9u & z.mask) };
 }
+// the rounding seed of direction d for a value with bits `bits`:
+// (bits & mask) ^ ditherDir(z, d).salt, formed with gfx950's three-input
+// bitwise instruction (one instruction when the mask is known to be all
+// ones, as after ditherOn, two otherwise, instead of three)
+__device__ __forceinline__ uint32_t seedOf(uint32_t bits, Dither z, int d)
+{
+  const uint32_t cd = ((uint32_t)d + 1u) * 0x9e3779b9u;
+  if (__builtin_constant_p(z.mask) && z.mask == 0u)
+    return z.salt; // rounding to nearest: the seed is the salt alone
+  if (__builtin_constant_p(z.mask) && z.mask == 0xffffffffu)
+    return __builtin_amdgcn_bitop3_b32(bits, z.salt, cd, 0x96); // a ^ b ^ c
+  return __builtin_amdgcn_bitop3_b32(bits, z.mask, z.salt ^ (cd & z.mask), 0x6a); // (a & b) ^ c
+}
 
 typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 typedef __bf16 bf162v __attribute__((ext_vector_type(2)));
@@ -177,6 +190,7 @@ struct Store<double> {
   __device__ static double storeStep(R v, R, Dither) { return v; }
   __device__ static R raw(double v) { return v; }
   __device__ static double rawStep(R v, Dither) { return v; }
+  __device__ static double rawStepD(R v, Dither, int) { return v; }
 };
 template <>
 struct Store<float> {
@@ -187,6 +201,7 @@ struct Store<float> {
   __device__ static float storeStep(R v, R, Dither) { return v; }
   __device__ static R raw(float v) { return v; }
   __device__ static float rawStep(R v, Dither) { return v; }
+  __device__ static float rawStepD(R v, Dither, int) { return v; }
 };
 // FP32 shifted-deviation storage: same bytes as float, scaled
 struct f32dev {
@@ -201,6 +216,7 @@ struct Store<f32dev> {
   __device__ static f32dev storeStep(R v, R sc, Dither) { return f32dev{ v * sc }; }
   __device__ static R raw(f32dev v) { return v.v; }
   __device__ static f32dev rawStep(R v, Dither) { return f32dev{ v }; }
+  __device__ static f32dev rawStepD(R v, Dither, int) { return f32dev{ v }; }
 };
 template <>
 struct Store<__half> {
@@ -218,6 +234,13 @@ struct Store<__half> {
     const half2v r      = __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(half2v, x), x, seed, false);
     return __half(__half_raw{ r[0] });
   }
+  // rawStep(x, ditherDir(z, d)), the seed formed by seedOf
+  __device__ static __half rawStepD(R x, Dither z, int d)
+  {
+    const uint32_t seed = seedOf(__float_as_uint(x), z, d);
+    const half2v r      = __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(half2v, x), x, seed, false);
+    return __half(__half_raw{ r[0] });
+  }
 };
 template <>
 struct Store<__hip_bfloat16> {
@@ -230,6 +253,13 @@ struct Store<__hip_bfloat16> {
   __device__ static __hip_bfloat16 rawStep(R x, Dither z)
   {
     const uint32_t seed = (__float_as_uint(x) & z.mask) ^ z.salt;
+    const bf162v r      = __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(bf162v, x), x, seed, false);
+    return __hip_bfloat16(__hip_bfloat16_raw{ __builtin_bit_cast(unsigned short, r[0]) });
+  }
+  // rawStep(x, ditherDir(z, d)), the seed formed by seedOf
+  __device__ static __hip_bfloat16 rawStepD(R x, Dither z, int d)
+  {
+    const uint32_t seed = seedOf(__float_as_uint(x), z, d);
     const bf162v r      = __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(bf162v, x), x, seed, false);
     return __hip_bfloat16(__hip_bfloat16_raw{ __builtin_bit_cast(unsigned short, r[0]) });
   }

@@ -124,7 +124,7 @@ __device__ __forceinline__ void storeRow(S *o, uint64_t plane, uint64_t rowCell,
 {
 #pragma unroll
   for (int d = 0; d < Q; ++d)
-    putS<NTS>(rowSlot(o, INV ? inv(d) : d, plane, rowCell, z), Store<S>::rawStep(f[d], ditherDir(dz, d)));
+    putS<NTS>(rowSlot(o, INV ? inv(d) : d, plane, rowCell, z), Store<S>::rawStepD(f[d], dz, d));
 }
 
 // Closing steps.  The value a push or AA odd step streams to x + c_d is

@@ -129,7 +129,7 @@ __global__ __launch_bounds__(512, LBM_MIN_WAVES) void kStepRow(S *a, S *b, const
           }
         }
       }
-      putS<NTS>(ptr, St::rawStep(fv, ditherDir(dz, d)));
+      putS<NTS>(ptr, St::rawStepD(fv, dz, d));
     }
   }
 }
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(256) void kStepAoS(const S *a, S *b, const MaskWin 
     if (lane < nvalid) {
 #pragma unroll
       for (int d = 0; d < Q; ++d)
-        st[lane * Q + d] = St::rawStep(f[d], ditherDir(dz, d));
+        st[lane * Q + d] = St::rawStepD(f[d], dz, d);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(256) void kStepAoS(const S *a, S *b, const MaskWin 
         i = bb ? (c0 + (uint64_t)lane) * Q + (uint64_t)inv(d)
                : (rowOf(cx(d), cy(d)) * p.nz + (uint64_t)zs[cz(d) + 1]) * Q + (uint64_t)d;
       }
-      putS<false>(&b[i], St::rawStep(f[d], ditherDir(dz, d)));
+      putS<false>(&b[i], St::rawStepD(f[d], dz, d));
     }
   }
 }
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(256) void kPushCollideAoS(const S *a, S *post, cons
   if (lane < nvalid) {
 #pragma unroll
     for (int d = 0; d < Q; ++d)
-      putS<NTS>(&post[(uint64_t)d * p.plane + c0 + (uint64_t)lane], St::rawStep(f[d], ditherDir(dz, d)));
+      putS<NTS>(&post[(uint64_t)d * p.plane + c0 + (uint64_t)lane], St::rawStepD(f[d], dz, d));
   }
 }
 
@@ -488,7 +488,9 @@ __device__ __forceinline__ S laneValue(S v, int lane)
   }
 }
 
-template <int V, int NT, typename S, bool CL = false>
+// DI: stochastic rounding on (FP16/BF16 with the rounding flag set), as a
+// compile-time switch, so that each direction's seed is one three-way xor
+template <int V, int NT, typename S, bool CL = false, bool DI = false>
 __global__ __launch_bounds__(256, LBM_MIN_WAVES) void kRowSweep(S *a, S *b,
     const MaskWin *__restrict__ mask, Params<S> p, int remap, int tiled)
 {
@@ -646,7 +648,7 @@ __global__ __launch_bounds__(256, LBM_MIN_WAVES) void kRowSweep(S *a, S *b,
       if (p.addForce)
         addForce(f, p.f1, p.f2);
       collide(f, p.trt);
-      const Dither dz = ditherOf(cell, seqv, p.dither);
+      const Dither dz = DI ? ditherOn(cell, seqv) : ditherNone();
 
 #pragma unroll
       for (int d = 0; d < Q; ++d) {
@@ -655,13 +657,13 @@ __global__ __launch_bounds__(256, LBM_MIN_WAVES) void kRowSweep(S *a, S *b,
         if constexpr (CL) {
           if (cx(d) != 0) {
             const R t = closeTerm(d, p.cf1, p.cf2);
-            v         = St::rawStep(f[d] + t, ditherDir(dz, d));
-            vb        = St::rawStep(f[d] - t, ditherDir(dz, d));
+            v         = St::rawStepD(f[d] + t, dz, d);
+            vb        = St::rawStepD(f[d] - t, dz, d);
           } else {
-            v = vb = St::rawStep(f[d], ditherDir(dz, d));
+            v = vb = St::rawStepD(f[d], dz, d);
           }
         } else {
-          v = vb = St::rawStep(f[d], ditherDir(dz, d));
+          v = vb = St::rawStepD(f[d], dz, d);
         }
         if (d == C) {
           if (fluid)
@@ -874,7 +876,7 @@ __global__ __launch_bounds__(512) void kRowGroup(S *a, S *b, const MaskWin *__re
       if (cx(d) != 0)
         fv += closeTerm(d, p.cf1, p.cf2);
     }
-    v[d] = St::rawStep(fv, ditherDir(dz, d));
+    v[d] = St::rawStepD(fv, dz, d);
     if ((cz(d) > 0 && lane == 63) || (cz(d) < 0 && lane == 0))
       xs[w][d - E0] = v[d];
   }
@@ -882,7 +884,7 @@ __global__ __launch_bounds__(512) void kRowGroup(S *a, S *b, const MaskWin *__re
   auto bbValue = [&](int d) -> S {
     if constexpr (CL) {
       if (cx(d) != 0)
-        return St::rawStep(f[d] - closeTerm(d, p.cf1, p.cf2), ditherDir(dz, d));
+        return St::rawStepD(f[d] - closeTerm(d, p.cf1, p.cf2), dz, d);
     }
     return v[d];
   };
@@ -941,7 +943,8 @@ __global__ __launch_bounds__(512) void kRowGroup(S *a, S *b, const MaskWin *__re
 // The loads complete before the first barrier (their values reach LDS or
 // the collision before the second) and every store follows the second, so
 // the slots that move between wavefronts are read before they are written.
-template <int NT, typename S, int WB = 8, bool CL = false>
+// DI: stochastic rounding on, as a compile-time switch (as in kRowSweep)
+template <int NT, typename S, int WB = 8, bool CL = false, bool DI = false>
 __global__ __launch_bounds__(512) void kRowGroupV(S *a, const MaskWin *__restrict__ mask,
     Params<S> p, int remap)
 {
@@ -1179,7 +1182,7 @@ __global__ __launch_bounds__(512) void kRowGroupV(S *a, const MaskWin *__restric
 #pragma unroll
     for (int d = 0; d < Q; ++d)
       asm("" : "+v"(f[d]));
-    const Dither dz = ditherOf(rc + (uint64_t)(z0 + j), seqv, p.dither);
+    const Dither dz = DI ? ditherOn(rc + (uint64_t)(z0 + j), seqv) : ditherOf(rc + (uint64_t)(z0 + j), seqv, 0);
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
       R fv = f[d], fb = f[d];
@@ -1189,10 +1192,10 @@ __global__ __launch_bounds__(512) void kRowGroupV(S *a, const MaskWin *__restric
           fb -= closeTerm(d, p.cf1, p.cf2);
         }
       }
-      const E ev = __builtin_bit_cast(E, St::rawStep(fv, ditherDir(dz, d)));
+      const E ev = __builtin_bit_cast(E, St::rawStepD(fv, dz, d));
       out[d] = (out[d] & ~((Wd)(E)~E(0) << (BITS * j))) | ((Wd)ev << (BITS * j));
       if constexpr (CL) {
-        const E eb = __builtin_bit_cast(E, St::rawStep(fb, ditherDir(dz, d)));
+        const E eb = __builtin_bit_cast(E, St::rawStepD(fb, dz, d));
         bb[d] |= (Wd)eb << (BITS * j);
       }
     }
@@ -1283,13 +1286,22 @@ static void launchRowGroupV(const StepLaunch &L, hipStream_t s)
   Params<S> p           = makeParams<S>(L);
   const unsigned blocks = (unsigned)((uint64_t)L.g.nx * L.g.ny);
   const unsigned lanes  = (unsigned)(L.g.nz / (WB / (int)sizeof(S))); // 64 per wavefront
-  if (L.close) {
-    hipLaunchKernelGGL((kRowGroupV<1, S, WB, true>), dim3(blocks), dim3(lanes), 0, s,
-        (S *)L.src, L.g.mask, p, L.remap);
-    return;
+  constexpr bool SR = std::is_same<S, __half>::value || std::is_same<S, __hip_bfloat16>::value;
+#define LAUNCH_RGV(NT_, CL_, DI_)                                                               \
+  hipLaunchKernelGGL((kRowGroupV<NT_, S, WB, CL_, DI_>), dim3(blocks), dim3(lanes), 0, s,        \
+      (S *)L.src, L.g.mask, p, L.remap)
+  if (SR && p.dither) {
+    if (L.close)
+      LAUNCH_RGV(1, true, SR);
+    else
+      LAUNCH_RGV(NT, false, SR);
+  } else {
+    if (L.close)
+      LAUNCH_RGV(1, true, false);
+    else
+      LAUNCH_RGV(NT, false, false);
   }
-  hipLaunchKernelGGL((kRowGroupV<NT, S, WB>), dim3(blocks), dim3(lanes), 0, s, (S *)L.src,
-      L.g.mask, p, L.remap);
+#undef LAUNCH_RGV
 }
 
 template <int V, int NT, typename S>
@@ -1297,14 +1309,23 @@ static void launchRowSweep(const StepLaunch &L, hipStream_t s)
 {
   Params<S> p           = makeParams<S>(L);
   const unsigned blocks = L.tiled ? tileBlocks(L.g, 1) : (unsigned)(((uint64_t)L.g.nx * L.g.ny + 3) / 4);
-  if (L.close) {
-    constexpr int NTC = sizeof(S) == 8 ? 3 : 1;
-    hipLaunchKernelGGL((kRowSweep<V, NTC, S, true>), dim3(blocks), dim3(256), 0, s, (S *)L.src,
-        (S *)L.dst, L.g.mask, p, L.remap, L.tiled);
-    return;
+  constexpr bool SR = std::is_same<S, __half>::value || std::is_same<S, __hip_bfloat16>::value;
+  constexpr int NTC  = sizeof(S) == 8 ? 3 : 1;
+#define LAUNCH_RS(NT_, CL_, DI_)                                                                \
+  hipLaunchKernelGGL((kRowSweep<V, NT_, S, CL_, DI_>), dim3(blocks), dim3(256), 0, s, (S *)L.src, \
+      (S *)L.dst, L.g.mask, p, L.remap, L.tiled)
+  if (SR && p.dither) {
+    if (L.close)
+      LAUNCH_RS(NTC, true, SR);
+    else
+      LAUNCH_RS(NT, false, SR);
+  } else {
+    if (L.close)
+      LAUNCH_RS(NTC, true, false);
+    else
+      LAUNCH_RS(NT, false, false);
   }
-  hipLaunchKernelGGL((kRowSweep<V, NT, S>), dim3(blocks), dim3(256), 0, s, (S *)L.src,
-      (S *)L.dst, L.g.mask, p, L.remap, L.tiled);
+#undef LAUNCH_RS
 }
 
 template <int V, bool AOS, int NT, typename S>
