@@ -910,8 +910,21 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
   // 8-byte elements row group 8.40 ms, sweep 8.80, per-block 9.30; 2-byte
   // sweep 2.32, row group 3.75, in 8-byte words 2.50, in 4-byte words 2.31;
   // 4-byte per-block 3.98, row group 4.51, sweep 4.69, in 8-byte words
-  // 4.3.  Unset for 2-byte AA storage: chosen from the solids (below).
+  // 4.3.
   D->rowSweep      = envInt("LBM_ROW_SWEEP", D->esize == 8 ? 3 : (D->esize == 4 ? 0 : 1));
+  // The AA odd step of 2-byte storage, LBM_ROW_SWEEP unset: the row group of
+  // 4-byte words (kRowGroupV, two cells per lane) where its shape fits.
+  // Measured at 512^3 (FP16 odd step, words against the row sweep, by the
+  // share of 128-cell wavefronts near a solid): plain channel (0.8 %) 1.83
+  // against 2.08 ms; 2000 spheres (46 %) 2.16 against 2.30; 6000 spheres
+  // (83 %) 2.43-2.45 against 2.53-2.55; 15000 spheres (99 %) 2.82-2.85
+  // against 2.91-2.92.
+  if (envInt("LBM_ROW_SWEEP", -1) < 0 && D->esize == 2 && D->prop == PROP_AA &&
+      D->layout == LAYOUT_SOA && D->nz % 128 == 0 && D->nz <= 1024 && D->plane % 2 == 0 &&
+      D->plane < (1ull << 32)) {
+    D->rowSweep = 4;
+    printf("# AA odd step: row group of 4-byte words\n");
+  }
   D->seq           = 0;
   D->steps         = 0;
   D->dseqHost      = 0;
@@ -982,34 +995,6 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
       hipMemcpyDeviceToHost, D->stream));
   HIP_OK(hipStreamSynchronize(D->stream));
   HIP_OK(hipFree(dlat));
-
-  // The AA odd step of 2-byte storage, LBM_ROW_SWEEP unset: the row group of
-  // 4-byte words (kRowGroupV, two cells per lane) runs a wavefront with no
-  // solid nearby about as fast as the even step and one near a solid more
-  // slowly (it masks stores per cell); the row sweep depends less on the
-  // solids.  Measured at 512^3 (FP16 odd step, words against sweep, by the
-  // share of 128-cell wavefronts near a solid): plain channel (0.8 %) 1.83
-  // against 2.08 ms; 2000 spheres (46 %) 2.31-2.37 against 2.27-2.35; 6000
-  // spheres (83 %) 2.67 against 2.57; 15000 spheres (99 %) 3.0 against 2.9.
-  // Below 45 % the word row group runs.
-  if (envInt("LBM_ROW_SWEEP", -1) < 0 && D->esize == 2 && D->prop == PROP_AA &&
-      D->layout == LAYOUT_SOA && D->nz % 128 == 0 && D->nz <= 1024 && D->plane % 2 == 0 &&
-      D->plane < (1ull << 32)) {
-    const int waves = D->nz / 128;
-    uint64_t near   = 0;
-    for (size_t r = 0; r < (size_t)D->nx * D->ny; ++r)
-      for (int w = 0; w < waves; ++w) {
-        uint64_t any = 0;
-        for (int i = 0; i < 2; ++i)
-          any |= D->hmask[r * D->segs + (size_t)w * 2 + i].hi >> 63;
-        near += any;
-      }
-    const double frac = (double)near / ((double)D->nx * D->ny * waves);
-    if (frac < 0.45)
-      D->rowSweep = 4;
-    printf("# AA odd step: %s (%.1f %% of the 128-cell wavefronts near a solid)\n",
-        D->rowSweep == 4 ? "row group of 4-byte words" : "row sweep", 100.0 * frac);
-  }
 
   if (!ensureScratch())
     fail("the device scratch buffer has no address on this device");

@@ -925,6 +925,7 @@ __global__ __launch_bounds__(512) void kRowGroup(S *a, S *b, const MaskWin *__re
 }
 
 // ---- vector row group: the AA odd step of 4- and 2-byte elements ----------------
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 // kRowGroup's shape for elements narrower than 8 bytes.  Each lane holds
 // CPL = WB / sizeof(S) consecutive cells of the row as one word of WB
 // bytes (8, or 4 for 2-byte elements; cell z0 + j in bits
@@ -988,12 +989,13 @@ __global__ __launch_bounds__(512) void kRowGroupV(S *a, const MaskWin *__restric
   // byte offset, so an access costs no vector address arithmetic and all
   // of them share one offset register
   const uint32_t zoff = (uint32_t)z0 * (uint32_t)sizeof(S);
-  auto wordPtr = [&](int ox, int oy, int d) -> Wd * {
-    typedef __attribute__((address_space(1))) char gchar;
+  typedef __attribute__((address_space(1))) char gchar;
+  auto rowBase = [&](int ox, int oy, int d) -> gchar * {
     gchar *base = (gchar *)(a + sl(rowStart(ox, oy), d));
     asm("" : "+s"(base));
-    return (Wd *)(base + zoff);
+    return base;
   };
+  auto wordPtr = [&](int ox, int oy, int d) -> Wd * { return (Wd *)(rowBase(ox, oy, d) + zoff); };
   auto ld = [&](const Wd *ptr, auto nt) -> Wd {
     if constexpr (decltype(nt)::value)
       return __builtin_nontemporal_load(ptr);
@@ -1006,20 +1008,42 @@ __global__ __launch_bounds__(512) void kRowGroupV(S *a, const MaskWin *__restric
     else
       *ptr = v;
   };
-  // cells j with bit j of m set get their value from v (per cell where not all)
-  auto stMasked = [&](Wd *ptr, Wd v, uint32_t m) {
-    if (m == FULL) {
-      st(ptr, v);
-    } else if (m != 0u) {
+  // Near a solid the stores go through a buffer resource on the target
+  // row's slot plane (nZ elements): a lane that stores nothing takes an
+  // offset beyond the resource's end, and the hardware drops such a store,
+  // so the lane's cell mask m costs a select rather than a branch.  Lanes
+  // with only some of their cells to store (bit j of m: cell z0 + j) store
+  // those cells one by one, in stores made only when the wavefront has
+  // such a lane.
+  constexpr uint32_t OOB = 0x80000000u; // beyond any row's bytes
+  constexpr int AUX      = NTS ? 2 : 0; // non-temporal
+  auto stNear = [&](int ox, int oy, int d, Wd v, uint32_t m) {
+    if (!__builtin_amdgcn_ballot_w64(m != 0u))
+      return;
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)rowBase(ox, oy, d), (short)0, nz * (int)sizeof(S), 0x00020000);
+    const uint32_t off = m == FULL ? zoff : OOB;
+    if constexpr (WB == 8)
+      __builtin_amdgcn_raw_buffer_store_b64(
+          u32x2{(uint32_t)v, (uint32_t)(v >> 32)}, r, (int)off, 0, AUX);
+    else
+      __builtin_amdgcn_raw_buffer_store_b32((uint32_t)v, r, (int)off, 0, AUX);
+    if (__builtin_amdgcn_ballot_w64(m != 0u && m != FULL)) {
+      auto stCell = [&](Wd e, uint32_t o) {
+        if constexpr (sizeof(S) == 2)
+          __builtin_amdgcn_raw_buffer_store_b16((unsigned short)e, r, (int)o, 0, AUX);
+        else
+          __builtin_amdgcn_raw_buffer_store_b32((uint32_t)e, r, (int)o, 0, AUX);
+      };
+      if constexpr (CPL == 2) {
+        // a split word has one cell to store: one store for both cells
+        const bool hi = m == 2u;
+        stCell(hi ? v >> BITS : v, m == 1u || hi ? zoff + (hi ? (uint32_t)sizeof(S) : 0u) : OOB);
+      } else {
 #pragma unroll
-      for (int j = 0; j < CPL; ++j) {
-        if ((m >> j) & 1u) {
-          const E e = (E)(v >> (BITS * j));
-          if constexpr (NTS)
-            __builtin_nontemporal_store(e, (E *)ptr + j);
-          else
-            ((E *)ptr)[j] = e;
-        }
+        for (int j = 0; j < CPL; ++j)
+          stCell(v >> (BITS * j),
+              m != FULL && ((m >> j) & 1u) ? zoff + (uint32_t)(j * (int)sizeof(S)) : OOB);
       }
     }
   };
@@ -1222,22 +1246,25 @@ __global__ __launch_bounds__(512) void kRowGroupV(S *a, const MaskWin *__restric
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
       if (d == C) {
-        stMasked(wordPtr(0, 0, C), out[C], fl);
+        if (W)
+          stNear(0, 0, C, out[C], fl);
+        else
+          st(wordPtr(0, 0, C), out[C]);
         continue;
       }
-      if (W) {
-        // bounce-back: fluid cells whose target x + c_d is solid, own slot inv d
-        const uint32_t m = fl & cellsAt(field(cx(d), cy(d)), cz(d));
-        stMasked(wordPtr(0, 0, inv(d)), bb[d], m);
-      }
+      // bounce-back: fluid cells whose target x + c_d is solid, own slot inv d
+      if (W)
+        stNear(0, 0, inv(d), bb[d], fl & cellsAt(field(cx(d), cy(d)), cz(d)));
       Wd sv = out[d];
       if (cz(d) > 0)
         sv = moveUp(sv, topOfBelow(sv, xs[wl][d - E0]));
       else if (cz(d) < 0)
         sv = moveDown(sv, bottomOfAbove(sv, xs[wr][d - E0]));
       // producers of the word's cells: z0 + j - c_z, skipped where solid
-      const uint32_t m = W ? FULL & ~cellsAt(field(0, 0), -cz(d)) : FULL;
-      stMasked(wordPtr(cx(d), cy(d), d), sv, m);
+      if (W)
+        stNear(cx(d), cy(d), d, sv, FULL & ~cellsAt(field(0, 0), -cz(d)));
+      else
+        st(wordPtr(cx(d), cy(d), d), sv);
     }
   };
   if (clean)

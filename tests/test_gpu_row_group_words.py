@@ -69,13 +69,15 @@ def test_row_group_words_match_per_block_kernel(gpu_lib, oracle, monkeypatch, fm
     assert diff.max() <= 64 * U[fmt] * dev, (fmt, nz, float(diff.max()), dev)
 
 
-@pytest.mark.parametrize("geo,per", [("fluid", (1, 1, 1)), ("spheres-3", (0, 0, 1))])
-def test_default_choice_follows_the_solids(gpu_lib, oracle, monkeypatch, capfd, geo, per):
+@pytest.mark.parametrize("geo,per,nz", [("fluid", (1, 1, 1), 512), ("spheres-3", (0, 0, 1), 512),
+                                        ("spheres-3", (0, 0, 1), 192)])
+def test_default_choice(gpu_lib, oracle, monkeypatch, capfd, geo, per, nz):
     """With LBM_ROW_SWEEP unset, 2-byte AA storage takes the row group of
-    4-byte words when fewer than 45 % of the 128-cell wavefronts are near a
-    solid and the row sweep otherwise; Init says which, and the run equals
-    the run with that kernel named explicitly, bit for bit"""
-    dims = (6, 8, 512)
+    4-byte words where its shape fits (nZ a multiple of 128 up to 1024),
+    whatever the solids, and the row sweep otherwise; Init says when it
+    takes the words, and the run equals the run with that kernel named
+    explicitly, bit for bit"""
+    dims = (6, 8, nz)
     lat, _, _ = oracle.geometry(geo, dims, per, 3)
     init = oracle.init_random(dims, lat, 5, 1e-2)
 
@@ -94,10 +96,7 @@ def test_default_choice_follows_the_solids(gpu_lib, oracle, monkeypatch, capfd, 
     capfd.readouterr()
     auto = run(None)
     line = [ln for ln in capfd.readouterr().out.splitlines() if ln.startswith("# AA odd step:")]
-    assert len(line) == 1, line
-    frac = float(line[0].split("(")[1].split("%")[0])
-    words = frac < 45.0
-    assert ("row group of 4-byte words" in line[0]) == words, line[0]
-    assert words == (geo == "fluid"), (geo, frac)
+    words = nz % 128 == 0
+    assert line == (["# AA odd step: row group of 4-byte words"] if words else []), line
     named = run("4" if words else "1")
     assert np.array_equal(auto.view(np.uint64), named.view(np.uint64))
