@@ -1013,15 +1013,16 @@ __global__ __launch_bounds__(512) void kRowGroupV(S *a, const MaskWin *__restric
   // offset beyond the resource's end, and the hardware drops such a store,
   // so the lane's cell mask m costs a select rather than a branch.  Lanes
   // with only some of their cells to store (bit j of m: cell z0 + j) store
-  // those cells one by one, in stores made only when the wavefront has
-  // such a lane.
+  // those cells one by one (in one store where a word holds two cells),
+  // made only when the wavefront has such a lane.
   constexpr uint32_t OOB = 0x80000000u; // beyond any row's bytes
   constexpr int AUX      = NTS ? 2 : 0; // non-temporal
+  constexpr int WORD3    = 0x00020000;  // untyped access, 32-bit data format (gfx9 family)
   auto stNear = [&](int ox, int oy, int d, Wd v, uint32_t m) {
     if (!__builtin_amdgcn_ballot_w64(m != 0u))
       return;
     const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
-        (void *)rowBase(ox, oy, d), (short)0, nz * (int)sizeof(S), 0x00020000);
+        (void *)rowBase(ox, oy, d), (short)0, nz * (int)sizeof(S), WORD3);
     const uint32_t off = m == FULL ? zoff : OOB;
     if constexpr (WB == 8)
       __builtin_amdgcn_raw_buffer_store_b64(
