@@ -1011,16 +1011,20 @@ __global__ __launch_bounds__(512) void kRowGroupV(S *a, const MaskWin *__restric
   // Near a solid the stores go through a buffer resource on the target
   // row's slot plane (nZ elements): a lane that stores nothing takes an
   // offset beyond the resource's end, and the hardware drops such a store,
-  // so the lane's cell mask m costs a select rather than a branch.  Lanes
-  // with only some of their cells to store (bit j of m: cell z0 + j) store
-  // those cells one by one (in one store where a word holds two cells),
-  // made only when the wavefront has such a lane.
+  // so the lane's cell mask m costs a select rather than a branch.  A lane
+  // whose word holds two cells and has one of them to store does so in a
+  // second store, which every lane makes (out of range where it has no
+  // such cell): without a branch on the wavefront's masks it ran faster
+  // than with one.  Words of four cells store their cells one by one, only
+  // where the wavefront has a lane with some but not all of its cells.
   constexpr uint32_t OOB = 0x80000000u; // beyond any row's bytes
   constexpr int AUX      = NTS ? 2 : 0; // non-temporal
   constexpr int WORD3    = 0x00020000;  // untyped access, 32-bit data format (gfx9 family)
   auto stNear = [&](int ox, int oy, int d, Wd v, uint32_t m) {
-    if (!__builtin_amdgcn_ballot_w64(m != 0u))
-      return;
+    if constexpr (CPL > 2) {
+      if (!__builtin_amdgcn_ballot_w64(m != 0u))
+        return;
+    }
     const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
         (void *)rowBase(ox, oy, d), (short)0, nz * (int)sizeof(S), WORD3);
     const uint32_t off = m == FULL ? zoff : OOB;
@@ -1029,23 +1033,20 @@ __global__ __launch_bounds__(512) void kRowGroupV(S *a, const MaskWin *__restric
           u32x2{(uint32_t)v, (uint32_t)(v >> 32)}, r, (int)off, 0, AUX);
     else
       __builtin_amdgcn_raw_buffer_store_b32((uint32_t)v, r, (int)off, 0, AUX);
-    if (__builtin_amdgcn_ballot_w64(m != 0u && m != FULL)) {
-      auto stCell = [&](Wd e, uint32_t o) {
-        if constexpr (sizeof(S) == 2)
-          __builtin_amdgcn_raw_buffer_store_b16((unsigned short)e, r, (int)o, 0, AUX);
-        else
-          __builtin_amdgcn_raw_buffer_store_b32((uint32_t)e, r, (int)o, 0, AUX);
-      };
-      if constexpr (CPL == 2) {
-        // a split word has one cell to store: one store for both cells
-        const bool hi = m == 2u;
-        stCell(hi ? v >> BITS : v, m == 1u || hi ? zoff + (hi ? (uint32_t)sizeof(S) : 0u) : OOB);
-      } else {
+    auto stCell = [&](Wd e, uint32_t o) {
+      if constexpr (sizeof(S) == 2)
+        __builtin_amdgcn_raw_buffer_store_b16((unsigned short)e, r, (int)o, 0, AUX);
+      else
+        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)e, r, (int)o, 0, AUX);
+    };
+    if constexpr (CPL == 2) {
+      const bool hi = m == 2u;
+      stCell(hi ? v >> BITS : v, m == 1u || hi ? zoff + (hi ? (uint32_t)sizeof(S) : 0u) : OOB);
+    } else if (__builtin_amdgcn_ballot_w64(m != 0u && m != FULL)) {
 #pragma unroll
-        for (int j = 0; j < CPL; ++j)
-          stCell(v >> (BITS * j),
-              m != FULL && ((m >> j) & 1u) ? zoff + (uint32_t)(j * (int)sizeof(S)) : OOB);
-      }
+      for (int j = 0; j < CPL; ++j)
+        stCell(v >> (BITS * j),
+            m != FULL && ((m >> j) & 1u) ? zoff + (uint32_t)(j * (int)sizeof(S)) : OOB);
     }
   };
   // one-cell moves of a word: toward higher z (cell z takes z - 1; `prev`
