@@ -773,15 +773,22 @@ static double probePlacement(LbmDevice *D, void *pdf)
 // took 15.3-15.4 ms for most and 14.0 ms for about one in four, and the
 // same pattern over the array itself (probePlacement) tells them apart
 // (5.5 against 5.9-6.0 TB/s).  For SoA arrays of at least 4 GiB, up to
-// LBM_PLACEMENT_PROBE candidates (default 6) are allocated side by side
-// while they fit in half the free device memory, each probed (three
-// launches of the pattern, about 20 ms at 512^3 FP64), and the fastest
-// kept.  (Stopping at the first candidate 4 % above the slowest kept a
+// LBM_PLACEMENT_PROBE candidates (default 16) are allocated side by side
+// while all of them together fit in half the device memory free before
+// the first (seven of a 512^3 FP64 array on a 288 GB device, sixteen of
+// an FP16 one), each probed (three launches of the pattern, about 20 ms
+// at 512^3 FP64), and the fastest kept.  More candidates raise the odds
+// of a fast placement; in 20-step bench runs paired on one box, 16
+// against 6 gave FP64 18,985-19,343 against 18,359-19,287 MLUPS and BF16
+// 67,978-68,629 against 64,646-68,364, but the low runs with 6 were each
+// box's first bench, so the difference is not shown beyond the spread.  (Stopping at the first candidate 4 % above the slowest kept a
 // 5.88 TB/s one whose steps ran between the two groups, so every
 // candidate is probed.)  The others are freed before the lattice is
 // initialised.
 static void *allocPlaced(LbmDevice *D, size_t bytes)
 {
+  size_t free0 = 0, total0 = 0;
+  HIP_OK(hipMemGetInfo(&free0, &total0));
   void *best = nullptr;
   HIP_OK(hipMalloc(&best, bytes));
   if (D->layout != LAYOUT_SOA || D->placementProbe < 2 || bytes < (4ull << 30) ||
@@ -792,7 +799,7 @@ static void *allocPlaced(LbmDevice *D, size_t bytes)
   for (int k = 1; k < D->placementProbe; ++k) {
     size_t freeB = 0, totalB = 0;
     HIP_OK(hipMemGetInfo(&freeB, &totalB));
-    if (bytes > freeB / 2)
+    if ((uint64_t)(k + 1) * bytes > free0 / 2 || bytes > freeB / 2)
       break;
     void *c = nullptr;
     if (hipMalloc(&c, bytes) != hipSuccess) {
@@ -898,7 +905,7 @@ static void initInternal(LatticeDescType *ld, KernelDataType **kdOut, CaseDataTy
   // the last step of a run adds its own body force at the store
   // (LBM_CLOSE_FUSED 0: a separate force pass after it, as before)
   D->closeFused    = envInt("LBM_CLOSE_FUSED", 1);
-  D->placementProbe      = envInt("LBM_PLACEMENT_PROBE", 6);
+  D->placementProbe      = envInt("LBM_PLACEMENT_PROBE", 16);
   D->placementCandidates = 1;
   D->placementGbs        = 0.0;
   // push / AA odd with nZ % 64 == 0: 1 the row sweep (one wavefront walks
